@@ -1,0 +1,351 @@
+// gmle_api.cpp -- the C ABI declared in include/gaussmle.h.
+//
+// Argument validation mirrors validate_fit_input (src/interface.jl:10-40) and
+// default_constraints (src/constraints.jl:16-47) of GaussMLE.jl.  The host-
+// pointer entry point replaces the GPU branch's per-batch allocate / copy /
+// launch / synchronise loop (src/interface.jl:220-262): it uploads the whole
+// call once, runs one launch per kernel shape and copies results back, keeping
+// device buffers cached between calls.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "../../include/gaussmle.h"
+#include "gmle_args.hpp"
+
+namespace gmle {
+
+#define GMLE_DECLARE_FINDER(MODEL, CAM, PREC) LaunchFn find_launcher_##MODEL##_##CAM##_##PREC(int box, int lanes);
+#define GMLE_DECLARE_ALL(MODEL)               \
+  GMLE_DECLARE_FINDER(MODEL, IDEAL, PREC_FP32) \
+  GMLE_DECLARE_FINDER(MODEL, IDEAL, PREC_MIXED) \
+  GMLE_DECLARE_FINDER(MODEL, SCMOS, PREC_FP32) \
+  GMLE_DECLARE_FINDER(MODEL, SCMOS, PREC_MIXED)
+GMLE_DECLARE_ALL(XYNB)
+GMLE_DECLARE_ALL(XYNBS)
+GMLE_DECLARE_ALL(XYNBSXSY)
+GMLE_DECLARE_ALL(ASTIG)
+
+LaunchFn find_launcher(int model, int camera, int box, int lanes, int precision) {
+#define GMLE_CASE(MODEL)                                                                   \
+  case MODEL:                                                                              \
+    if (camera == IDEAL)                                                                   \
+      return precision == PREC_FP32 ? find_launcher_##MODEL##_IDEAL_PREC_FP32(box, lanes)  \
+                                    : find_launcher_##MODEL##_IDEAL_PREC_MIXED(box, lanes); \
+    return precision == PREC_FP32 ? find_launcher_##MODEL##_SCMOS_PREC_FP32(box, lanes)    \
+                                  : find_launcher_##MODEL##_SCMOS_PREC_MIXED(box, lanes);
+  switch (model) {
+    GMLE_CASE(XYNB)
+    GMLE_CASE(XYNBS)
+    GMLE_CASE(XYNBSXSY)
+    GMLE_CASE(ASTIG)
+  }
+#undef GMLE_CASE
+  return nullptr;
+}
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  return fail(e == hipErrorOutOfMemory ? GMLE_ERR_NOMEM : GMLE_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+bool lanes_ok(int box, int lanes) {
+  return box >= kMinBox && box <= kMaxBox && ((lanes == 1 && box <= 12) || lanes == 2 || lanes == 4);
+}
+
+// Lanes per ROI when the caller passes 0.  One lane per ROI keeps the
+// per-ROI work free of cross-lane traffic and is the fastest once the launch
+// has enough waves to fill the chip; smaller batches trade that for more
+// waves (more lanes per ROI), and large boxes need more lanes to keep the ROI
+// resident in VGPRs.
+int default_lanes(int model, int camera, int box, int precision, int64_t n_rois) {
+  (void)model;
+  (void)camera;
+  (void)precision;
+  const int64_t fill = 256 * 4 * 2 * 64;   // CUs x SIMDs x waves x lanes
+  int lanes = box <= 12 ? 1 : 2;
+  if (n_rois * lanes < fill && lanes_ok(box, 2)) lanes = 2;
+  if (n_rois * lanes < fill && lanes_ok(box, 4)) lanes = 4;
+  return lanes;
+}
+
+// Shared argument checks of both entry points.
+int check_common(int model, const float* psf, int box, int64_t n_rois, int camera, int ny, int nx,
+                 const float* lower, const float* upper, const float* max_step, int iterations, int precision,
+                 int* lanes) {
+  if (model < XYNB || model > ASTIG) return fail(GMLE_ERR_ARG, "unknown PSF model %d", model);
+  if (camera != IDEAL && camera != SCMOS) return fail(GMLE_ERR_ARG, "unknown camera model %d", camera);
+  if (precision != PREC_FP32 && precision != PREC_MIXED)
+    return fail(GMLE_ERR_ARG, "unknown precision %d", precision);
+  if (n_rois <= 0) return fail(GMLE_ERR_ARG, "Input data array is empty");
+  if (box < kMinBox) return fail(GMLE_ERR_ARG, "ROI size must be at least 3x3, got %dx%d", box, box);
+  if (box > kMaxBox) return fail(GMLE_ERR_UNSUPPORTED, "ROI size %d exceeds the supported maximum %d", box, kMaxBox);
+  if (iterations < 0) return fail(GMLE_ERR_ARG, "iterations must be >= 0, got %d", iterations);
+  if (!lower || !upper || !max_step) return fail(GMLE_ERR_ARG, "constraints arrays must not be NULL");
+  if ((model == XYNB || model == ASTIG) && !psf) return fail(GMLE_ERR_ARG, "psf parameters must not be NULL");
+  if (model == XYNB && !(psf[0] > 0.f)) return fail(GMLE_ERR_ARG, "PSF sigma must be > 0");
+  if (model == ASTIG && !(psf[0] > 0.f && psf[1] > 0.f && psf[7] != 0.f && psf[6] != 0.f))
+    return fail(GMLE_ERR_ARG, "astigmatic PSF needs sigma_x0, sigma_y0 > 0 and nonzero gamma, d");
+  if (camera == SCMOS && (ny < box || nx < box))
+    return fail(GMLE_ERR_ARG, "variance map %dx%d is smaller than one ROI", ny, nx);
+  if (*lanes == 0) *lanes = default_lanes(model, camera, box, precision, n_rois);
+  if (!lanes_ok(box, *lanes))
+    return fail(GMLE_ERR_UNSUPPORTED, "no kernel with %d lanes per ROI for box size %d", *lanes, box);
+  return GMLE_OK;
+}
+
+void fill_args(FitArgs& a, int model, const float* psf, const float* lower, const float* upper,
+               const float* max_step, int64_t n_rois, int ny, int nx, int iterations) {
+  std::memset(&a, 0, sizeof(a));
+  const int P = n_params(model);
+  for (int q = 0; q < 8; ++q) a.psf[q] = 0.f;
+  if (model == XYNB) a.psf[0] = psf[0];
+  if (model == ASTIG)
+    for (int q = 0; q < 8; ++q) a.psf[q] = psf[q];
+  for (int k = 0; k < P; ++k) {
+    a.lower[k] = lower[k];
+    a.upper[k] = upper[k];
+    a.max_step[k] = max_step[k];
+  }
+  a.n_rois = n_rois;
+  a.ny = ny;
+  a.nx = nx;
+  a.iterations = iterations;
+}
+
+// Device buffers cached across host-pointer calls (grow only).
+struct Cache {
+  std::mutex mu;
+  int device = -1;
+  void* buf[8] = {};
+  size_t cap[8] = {};
+  hipStream_t stream = nullptr;
+};
+Cache g_cache;
+
+hipError_t ensure(int slot, size_t bytes) {
+  if (g_cache.cap[slot] >= bytes) return hipSuccess;
+  if (g_cache.buf[slot]) (void)hipFree(g_cache.buf[slot]);
+  g_cache.buf[slot] = nullptr;
+  g_cache.cap[slot] = 0;
+  hipError_t e = hipMalloc(&g_cache.buf[slot], bytes);
+  if (e == hipSuccess) g_cache.cap[slot] = bytes;
+  return e;
+}
+
+}  // namespace
+}  // namespace gmle
+
+using namespace gmle;
+
+extern "C" {
+
+const char* gmle_version(void) { return "gaussmle-hip 0.1.0 (gfx950)"; }
+
+int gmle_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return n;
+}
+
+size_t gmle_last_error(char* buf, size_t len) {
+  if (buf && len) {
+    const size_t n = g_err.size() < len - 1 ? g_err.size() : len - 1;
+    std::memcpy(buf, g_err.data(), n);
+    buf[n] = '\0';
+  }
+  return g_err.size();
+}
+
+int gmle_n_params(int model) {
+  if (model < XYNB || model > ASTIG) return fail(GMLE_ERR_ARG, "unknown PSF model %d", model);
+  return n_params(model);
+}
+
+int gmle_default_constraints(int model, int box, float* lower, float* upper, float* max_step) {
+  if (model < XYNB || model > ASTIG) return fail(GMLE_ERR_ARG, "unknown PSF model %d", model);
+  if (!lower || !upper || !max_step) return fail(GMLE_ERR_ARG, "output arrays must not be NULL");
+  const float inf = INFINITY;
+  const float hi = (float)box + 2.f;
+  // src/constraints.jl:16-47
+  switch (model) {
+    case XYNB: {
+      const float lo[4] = {-2.f, -2.f, 1.f, 0.01f}, up[4] = {hi, hi, inf, inf}, ms[4] = {1.f, 1.f, 100.f, 2.f};
+      for (int k = 0; k < 4; ++k) { lower[k] = lo[k]; upper[k] = up[k]; max_step[k] = ms[k]; }
+      break;
+    }
+    case XYNBS: {
+      const float lo[5] = {-2.f, -2.f, 1.f, 0.01f, 0.3f}, up[5] = {hi, hi, inf, inf, 5.f};
+      const float ms[5] = {1.f, 1.f, 100.f, 2.f, 0.5f};
+      for (int k = 0; k < 5; ++k) { lower[k] = lo[k]; upper[k] = up[k]; max_step[k] = ms[k]; }
+      break;
+    }
+    case XYNBSXSY: {
+      const float lo[6] = {-2.f, -2.f, 1.f, 0.01f, 0.3f, 0.3f}, up[6] = {hi, hi, inf, inf, 5.f, 5.f};
+      const float ms[6] = {1.f, 1.f, 100.f, 2.f, 0.5f, 0.5f};
+      for (int k = 0; k < 6; ++k) { lower[k] = lo[k]; upper[k] = up[k]; max_step[k] = ms[k]; }
+      break;
+    }
+    default: {
+      const float lo[5] = {-2.f, -2.f, -1.f, 1.f, 0.01f}, up[5] = {hi, hi, 1.f, inf, inf};
+      const float ms[5] = {1.f, 1.f, 0.1f, 100.f, 2.f};
+      for (int k = 0; k < 5; ++k) { lower[k] = lo[k]; upper[k] = up[k]; max_step[k] = ms[k]; }
+      break;
+    }
+  }
+  return GMLE_OK;
+}
+
+int gmle_max_box(void) { return kMaxBox; }
+
+int gmle_lanes_supported(int box, int lanes) { return lanes_ok(box, lanes) ? 1 : 0; }
+
+int gmle_default_lanes(int model, int camera, int box, int precision, int64_t n_rois) {
+  return default_lanes(model, camera, box, precision, n_rois);
+}
+
+int gmle_fit_device(int model, const float* psf, int box, int64_t n_rois, const float* d_data, int camera,
+                    const float* d_var_map, int ny, int nx, const int32_t* d_x_corners,
+                    const int32_t* d_y_corners, const float* lower, const float* upper, const float* max_step,
+                    int iterations, int precision, int lanes, float* d_params, float* d_crlb, float* d_llr,
+                    double* d_crlb64, double* d_llr64, void* stream) {
+  int rc = check_common(model, psf, box, n_rois, camera, ny, nx, lower, upper, max_step, iterations, precision,
+                        &lanes);
+  if (rc != GMLE_OK) return rc;
+  if (!d_data || !d_params || !d_crlb || !d_llr) return fail(GMLE_ERR_ARG, "data / output pointers must not be NULL");
+  if (camera == SCMOS && (!d_var_map || !d_x_corners || !d_y_corners))
+    return fail(GMLE_ERR_ARG, "sCMOS fits need the variance map and ROI corners");
+  LaunchFn fn = find_launcher(model, camera, box, lanes, precision);
+  if (!fn) return fail(GMLE_ERR_UNSUPPORTED, "no kernel for model %d box %d lanes %d", model, box, lanes);
+  FitArgs a;
+  fill_args(a, model, psf, lower, upper, max_step, n_rois, ny, nx, iterations);
+  a.data = d_data;
+  a.var_map = d_var_map;
+  a.x_corners = d_x_corners;
+  a.y_corners = d_y_corners;
+  a.params = d_params;
+  a.crlb = d_crlb;
+  a.llr = d_llr;
+  a.crlb64 = precision == PREC_MIXED ? d_crlb64 : nullptr;
+  a.llr64 = precision == PREC_MIXED ? d_llr64 : nullptr;
+  const hipError_t e = fn(a, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "kernel launch");
+  return GMLE_OK;
+}
+
+int gmle_fit(int model, const float* psf, int box, int64_t n_rois, const float* data, int camera,
+             const float* var_map, int ny, int nx, const int32_t* x_corners, const int32_t* y_corners,
+             const float* lower, const float* upper, const float* max_step, int iterations, int precision,
+             int lanes, float* params, float* crlb, float* llr, double* crlb64, double* llr64) {
+  int rc = check_common(model, psf, box, n_rois, camera, ny, nx, lower, upper, max_step, iterations, precision,
+                        &lanes);
+  if (rc != GMLE_OK) return rc;
+  if (!data || !params || !crlb || !llr) return fail(GMLE_ERR_ARG, "data / output pointers must not be NULL");
+  const size_t npix = (size_t)n_rois * (size_t)box * (size_t)box;
+  // validate_fit_input (src/interface.jl:26-29): NaN / Inf rejected.
+  for (size_t q = 0; q < npix; ++q)
+    if (!std::isfinite(data[q])) return fail(GMLE_ERR_DATA, "Input data contains NaN or Inf values");
+  if (camera == SCMOS) {
+    if (!var_map || !x_corners || !y_corners)
+      return fail(GMLE_ERR_ARG, "sCMOS fits need the variance map and ROI corners");
+    for (int64_t k = 0; k < n_rois; ++k) {
+      const int cx = x_corners[k], cy = y_corners[k];
+      if (cx < 1 || cy < 1 || cx > nx - box + 1 || cy > ny - box + 1)
+        return fail(GMLE_ERR_BOUNDS, "ROI %lld at corner (x=%d, y=%d) lies outside the %dx%d variance map",
+                    (long long)(k + 1), cx, cy, ny, nx);
+    }
+  }
+  if (gmle_device_count() <= 0) return fail(GMLE_ERR_NODEVICE, "no HIP device available");
+  LaunchFn fn = find_launcher(model, camera, box, lanes, precision);
+  if (!fn) return fail(GMLE_ERR_UNSUPPORTED, "no kernel for model %d box %d lanes %d", model, box, lanes);
+
+  std::lock_guard<std::mutex> lock(g_cache.mu);
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  if (g_cache.device != dev) {
+    // Buffers belong to another device: drop them (not freed there, since that
+    // device may not be current) and start over on this one.
+    for (int s = 0; s < 8; ++s) { g_cache.buf[s] = nullptr; g_cache.cap[s] = 0; }
+    g_cache.stream = nullptr;
+    g_cache.device = dev;
+  }
+  if (!g_cache.stream) {
+    e = hipStreamCreateWithFlags(&g_cache.stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
+  }
+  const int P = n_params(model);
+  const size_t nvar = camera == SCMOS ? (size_t)ny * (size_t)nx : 0;
+  const bool want64 = precision == PREC_MIXED && (crlb64 || llr64);
+  if ((e = ensure(0, npix * sizeof(float))) != hipSuccess) return hip_fail(e, "hipMalloc data");
+  if ((e = ensure(1, (size_t)n_rois * P * sizeof(float))) != hipSuccess) return hip_fail(e, "hipMalloc params");
+  if ((e = ensure(2, (size_t)n_rois * P * sizeof(float))) != hipSuccess) return hip_fail(e, "hipMalloc crlb");
+  if ((e = ensure(3, (size_t)n_rois * sizeof(float))) != hipSuccess) return hip_fail(e, "hipMalloc llr");
+  if (camera == SCMOS) {
+    if ((e = ensure(4, nvar * sizeof(float))) != hipSuccess) return hip_fail(e, "hipMalloc var_map");
+    if ((e = ensure(5, (size_t)n_rois * 2 * sizeof(int32_t))) != hipSuccess) return hip_fail(e, "hipMalloc corners");
+  }
+  if (want64) {
+    if ((e = ensure(6, (size_t)n_rois * P * sizeof(double))) != hipSuccess) return hip_fail(e, "hipMalloc crlb64");
+    if ((e = ensure(7, (size_t)n_rois * sizeof(double))) != hipSuccess) return hip_fail(e, "hipMalloc llr64");
+  }
+  hipStream_t s = g_cache.stream;
+  float* d_data = (float*)g_cache.buf[0];
+  int32_t* d_corners = (int32_t*)g_cache.buf[5];
+  e = hipMemcpyAsync(d_data, data, npix * sizeof(float), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess && camera == SCMOS) {
+    e = hipMemcpyAsync(g_cache.buf[4], var_map, nvar * sizeof(float), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(d_corners, x_corners, (size_t)n_rois * sizeof(int32_t), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(d_corners + n_rois, y_corners, (size_t)n_rois * sizeof(int32_t), hipMemcpyHostToDevice, s);
+  }
+  if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync (upload)");
+  FitArgs a;
+  fill_args(a, model, psf, lower, upper, max_step, n_rois, ny, nx, iterations);
+  a.data = d_data;
+  a.var_map = camera == SCMOS ? (const float*)g_cache.buf[4] : nullptr;
+  a.x_corners = camera == SCMOS ? d_corners : nullptr;
+  a.y_corners = camera == SCMOS ? d_corners + n_rois : nullptr;
+  a.params = (float*)g_cache.buf[1];
+  a.crlb = (float*)g_cache.buf[2];
+  a.llr = (float*)g_cache.buf[3];
+  a.crlb64 = want64 ? (double*)g_cache.buf[6] : nullptr;
+  a.llr64 = want64 ? (double*)g_cache.buf[7] : nullptr;
+  e = fn(a, s);
+  if (e != hipSuccess) return hip_fail(e, "kernel launch");
+  e = hipMemcpyAsync(params, a.params, (size_t)n_rois * P * sizeof(float), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(crlb, a.crlb, (size_t)n_rois * P * sizeof(float), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(llr, a.llr, (size_t)n_rois * sizeof(float), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess && want64 && crlb64)
+    e = hipMemcpyAsync(crlb64, a.crlb64, (size_t)n_rois * P * sizeof(double), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess && want64 && llr64)
+    e = hipMemcpyAsync(llr64, a.llr64, (size_t)n_rois * sizeof(double), hipMemcpyDeviceToHost, s);
+  if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync (download)");
+  e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+  return GMLE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,52 @@
+// gmle_args.hpp -- kernel argument block and launch plumbing shared by the
+// kernel translation units and the C-ABI layer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gmle {
+
+enum Model { XYNB = 0, XYNBS = 1, XYNBSXSY = 2, ASTIG = 3 };
+enum Camera { IDEAL = 0, SCMOS = 1 };
+enum Precision { PREC_FP32 = 0, PREC_MIXED = 1 };
+
+constexpr int kMinBox = 3;
+constexpr int kMaxBox = 16;
+constexpr int kWave = 64;
+
+__host__ __device__ constexpr int n_params(int model) {
+  return model == XYNB ? 4 : model == XYNBS ? 5 : model == XYNBSXSY ? 6 : 5;
+}
+
+// Passed by value to every fitting kernel.  Arrays follow the reference's
+// Julia layout: data (box, box, n) column-major with ROI k at offset box*box*k;
+// params / crlb (P, n) with ROI k's P values contiguous; var_map (ny, nx)
+// column-major indexed [row = y, col = x]; corners 1-based.
+struct FitArgs {
+  const float* data;
+  const float* var_map;
+  const int32_t* x_corners;
+  const int32_t* y_corners;
+  float* params;
+  float* crlb;
+  float* llr;
+  double* crlb64;   // optional (mixed precision); may be null
+  double* llr64;    // optional (mixed precision); may be null
+  int64_t n_rois;
+  int ny, nx;
+  int iterations;
+  float psf[8];     // XYNB: [sigma]; ASTIG: [sx0, sy0, Ax, Ay, Bx, By, gamma, d] (pixel units)
+  float lower[6];
+  float upper[6];
+  float max_step[6];
+};
+
+// Host-side launcher of one (model, camera, box, lanes-per-ROI, precision)
+// specialisation; returns hipSuccess or the launch error.
+typedef hipError_t (*LaunchFn)(const FitArgs&, hipStream_t);
+
+// Lookup implemented by the per-model translation units.
+LaunchFn find_launcher(int model, int camera, int box, int lanes, int precision);
+
+}  // namespace gmle

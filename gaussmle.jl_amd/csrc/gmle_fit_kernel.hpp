@@ -1,0 +1,670 @@
+// gmle_fit_kernel.hpp -- the per-ROI maximum-likelihood fitting kernel for gfx950.
+//
+// One kernel specialisation per (PSF model, camera, box size N, lanes per ROI L,
+// precision).  It replaces GaussMLE.jl's unified_gaussian_mle_kernel!
+// (src/unified_kernel.jl:285-434) together with everything it inlines:
+// compute_pixel_derivatives (src/psf_derivatives.jl:6-187), the pixel-integrated
+// Gaussian and its derivatives (src/gausslib/baselibrary.jl:8-67), the camera
+// likelihood terms (src/camera_models.jl:19-96) and apply_constraints!
+// (src/constraints.jl:172-177).
+//
+// Work mapping (wave64, one wave per workgroup):
+//   * L lanes (1, 2 or 4, inside one quad) share one ROI; 64/L ROIs per wave.
+//   * Lane r of a group owns the contiguous column block [r*CB, r*CB + CB) with
+//     CB = ceil(N/L); its CB*N pixels (and sCMOS variances) are loaded once and
+//     stay in VGPRs for every Newton iteration and the final pass.
+//   * The model is separable: value = bg + N*Phix[j]*Phiy[i].  Per iteration
+//     each axis needs only N+1 edge evaluations (one erf and one exp each)
+//     instead of the reference's 4 erf + 4-8 exp per pixel; every per-pixel
+//     gradient / diagonal-Hessian term is a product of a column table and a row
+//     table, so the per-pixel work is one reciprocal plus about ten FMAs that
+//     feed per-column accumulators.  Column totals are folded into the P
+//     gradient and P diagonal-Hessian sums, which are summed across the L lanes
+//     with DPP quad permutes.
+//   * The Newton update (exact division, step clamp, box clamp) is replicated
+//     on the L lanes so they stay bit-identical.
+//   * PREC_FP32 computes the final pass (LLR, full Fisher matrix, inversion,
+//     CRLB) in Float32 like the reference; PREC_MIXED recomputes the model and
+//     derivatives at the final estimate in Float64 and does the LLR, Fisher
+//     accumulation and inversion in Float64.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include <type_traits>
+
+#include "gmle_args.hpp"
+#include "gmle_device.hpp"
+
+namespace gmle {
+
+// ---------------------------------------------------------------------------
+// Model traits.  EXTRA: 0 none (XYNB), 1 one sigma on both axes (XYNBS),
+// 2 separate sigma per axis (XYNBSXSY), 3 astigmatic z (chain rule through
+// sigma_x(z), sigma_y(z)).
+// ---------------------------------------------------------------------------
+template <int MODEL> struct MT;
+template <> struct MT<XYNB> {
+  static constexpr int P = 4, IX = 0, IY = 1, IN = 2, IBG = 3, IE1 = -1, IE2 = -1, EXTRA = 0;
+};
+template <> struct MT<XYNBS> {
+  static constexpr int P = 5, IX = 0, IY = 1, IN = 2, IBG = 3, IE1 = 4, IE2 = -1, EXTRA = 1;
+};
+template <> struct MT<XYNBSXSY> {
+  static constexpr int P = 6, IX = 0, IY = 1, IN = 2, IBG = 3, IE1 = 4, IE2 = 5, EXTRA = 2;
+};
+template <> struct MT<ASTIG> {
+  static constexpr int P = 5, IX = 0, IY = 1, IN = 3, IBG = 4, IE1 = 2, IE2 = -1, EXTRA = 3;
+};
+
+// ---------------------------------------------------------------------------
+// Scalar math per arithmetic type.
+// ---------------------------------------------------------------------------
+template <class T> struct DM;
+template <> struct DM<float> {
+  static __device__ __forceinline__ float erf_(float x) { return erff(x); }
+  // exp(-t^2 / (2 s^2)) given k = sqrt(log2(e)/2) / s
+  static __device__ __forceinline__ float gauss(float t, float k) {
+    const float a = t * k;
+    return fast_exp2(-(a * a));
+  }
+  static __device__ __forceinline__ float kexp(float inv_s) { return kSqrtHalfLog2e * inv_s; }
+  static __device__ __forceinline__ float inv_sqrt_2pi() { return kInvSqrt2Pi; }
+};
+template <> struct DM<double> {
+  static __device__ __forceinline__ double erf_(double x) { return erf(x); }
+  static __device__ __forceinline__ double gauss(double t, double inv_s) {
+    const double a = t * inv_s;
+    return exp(-0.5 * (a * a));
+  }
+  static __device__ __forceinline__ double kexp(double inv_s) { return inv_s; }
+  static __device__ __forceinline__ double inv_sqrt_2pi() { return kInvSqrt2PiD; }
+};
+
+// ---------------------------------------------------------------------------
+// Per-axis tables for M consecutive pixels starting at 0-based index `first`
+// (pixel k covers [k+1/2, k+3/2] in the reference's 1-based coordinates).  For
+// pixel p with lower edge lo and upper edge hi (t = edge - mu):
+//   phi = (erf(t_hi*s) - erf(t_lo*s)) / 2                  baselibrary.jl:8-14
+//   D   = c (g_hi - g_lo),  c = -N/(sqrt(2pi) sigma)        baselibrary.jl:21-33
+//   D2  = c/sigma^2 (t_hi g_hi - t_lo g_lo)
+//   S   = c/sigma   (t_hi g_hi - t_lo g_lo)                 baselibrary.jl:40-52
+//   SS  = -2/sigma S - N/(sqrt(2pi) sigma^5) (t_hi^3 g_hi - t_lo^3 g_lo)
+// with g = exp(-t^2/(2 sigma^2)).  Pixel-level derivatives are these times the
+// other axis' phi: dudt_x = D[j] phiy[i], d2udt2_x = D2[j] phiy[i], and the
+// sigma derivative of one axis is S[j] phiy[i] (second: SS[j] phiy[i]).
+// ---------------------------------------------------------------------------
+template <class T, int M> struct AxisTab {
+  T phi[M], D[M], D2[M], S[M], SS[M];
+};
+
+template <class T, int M, bool SIG>
+__device__ __forceinline__ void axis_tables(T mu, T sigma, T nph, int first, AxisTab<T, M>& tb) {
+  const T inv_s = T(1) / sigma;
+  const T s_erf = sqrt(T(0.5) / (sigma * sigma));
+  const T kx = DM<T>::kexp(inv_s);
+  const T c = -nph * DM<T>::inv_sqrt_2pi() * inv_s;
+  const T c2 = c * inv_s * inv_s;
+  const T cs = c * inv_s;
+  const T css = c2 * inv_s * inv_s;
+  const T m2s = T(-2) * inv_s;
+  T erf_prev = T(0), g_prev = T(0), tg_prev = T(0), t3g_prev = T(0);
+#pragma unroll
+  for (int e = 0; e <= M; ++e) {
+    const T t = (T(first + e) + T(0.5)) - mu;
+    const T ev = DM<T>::erf_(t * s_erf);
+    const T g = DM<T>::gauss(t, kx);
+    const T tg = t * g;
+    const T t3g = SIG ? t * t * tg : T(0);
+    if (e > 0) {
+      const int p = e - 1;
+      tb.phi[p] = T(0.5) * (ev - erf_prev);
+      tb.D[p] = c * (g - g_prev);
+      const T w = tg - tg_prev;
+      tb.D2[p] = c2 * w;
+      if (SIG) {
+        tb.S[p] = cs * w;
+        tb.SS[p] = m2s * tb.S[p] + css * (t3g - t3g_prev);
+      }
+    }
+    erf_prev = ev;
+    g_prev = g;
+    tg_prev = tg;
+    t3g_prev = t3g;
+  }
+}
+
+// Axis constants of axis_tables() and single-edge evaluation, for computing
+// one pixel's tables from its two edges on the fly (column side).
+template <class T> struct AxisK {
+  T mu, s_erf, kx, c, c2, cs, css, m2s;
+};
+template <class T> struct Edge {
+  T ev, g, tg, t3g;
+};
+template <class T>
+__device__ __forceinline__ AxisK<T> axis_consts(T mu, T sigma, T nph) {
+  AxisK<T> k;
+  const T inv_s = T(1) / sigma;
+  k.mu = mu;
+  k.s_erf = sqrt(T(0.5) / (sigma * sigma));
+  k.kx = DM<T>::kexp(inv_s);
+  k.c = -nph * DM<T>::inv_sqrt_2pi() * inv_s;
+  k.c2 = k.c * inv_s * inv_s;
+  k.cs = k.c * inv_s;
+  k.css = k.c2 * inv_s * inv_s;
+  k.m2s = T(-2) * inv_s;
+  return k;
+}
+template <class T, bool SIG>
+__device__ __forceinline__ Edge<T> edge_eval(const AxisK<T>& k, int e) {
+  Edge<T> r;
+  const T t = (T(e) + T(0.5)) - k.mu;
+  r.ev = DM<T>::erf_(t * k.s_erf);
+  r.g = DM<T>::gauss(t, k.kx);
+  r.tg = t * r.g;
+  r.t3g = SIG ? t * t * r.tg : T(0);
+  return r;
+}
+// One pixel's tables (see axis_tables) from its lower / upper edges.
+template <class T> struct Pix {
+  T phi, D, D2, S, SS;
+};
+template <class T, bool SIG>
+__device__ __forceinline__ Pix<T> pix_from_edges(const AxisK<T>& k, const Edge<T>& lo, const Edge<T>& hi) {
+  Pix<T> p;
+  p.phi = T(0.5) * (hi.ev - lo.ev);
+  p.D = k.c * (hi.g - lo.g);
+  const T w = hi.tg - lo.tg;
+  p.D2 = k.c2 * w;
+  p.S = SIG ? k.cs * w : T(0);
+  p.SS = SIG ? k.m2s * p.S + k.css * (hi.t3g - lo.t3g) : T(0);
+  return p;
+}
+
+// Per-ROI width parameters for one evaluation of the model.
+template <class T> struct Widths {
+  T sx, sy;          // sigma per axis (pixels)
+  T dsx, dsy;        // astig: d sigma / dz
+  T d2sx, d2sy;      // astig: d2 sigma / dz2
+};
+
+// psf_derivatives.jl:115-163: sigma_x(z), sigma_y(z) and their z derivatives.
+template <class T>
+__device__ __forceinline__ void astig_widths(T z, const float* psf, Widths<T>& w) {
+  const T sx0 = (T)psf[0], sy0 = (T)psf[1], Ax = (T)psf[2], Ay = (T)psf[3];
+  const T Bx = (T)psf[4], By = (T)psf[5], gamma = (T)psf[6], dd = (T)psf[7];
+  const T zdx = (z - gamma) / dd;
+  const T zdy = (z + gamma) / dd;
+  const T zdx2 = zdx * zdx, zdy2 = zdy * zdy;
+  const T ax = T(1) + zdx2 + Ax * (zdx2 * zdx) + Bx * (zdx2 * zdx2);
+  const T ay = T(1) + zdy2 + Ay * (zdy2 * zdy) + By * (zdy2 * zdy2);
+  const T sqax = sqrt(ax), sqay = sqrt(ay);
+  w.sx = sx0 * sqax;
+  w.sy = sy0 * sqay;
+  const T dax = (T(2) * zdx + T(3) * Ax * zdx2 + T(4) * Bx * (zdx2 * zdx)) / dd;
+  const T day = (T(2) * zdy + T(3) * Ay * zdy2 + T(4) * By * (zdy2 * zdy)) / dd;
+  const T hx = sx0 / (T(2) * sqax), hy = sy0 / (T(2) * sqay);
+  w.dsx = hx * dax;
+  w.dsy = hy * day;
+  const T d2ax = (T(2) + T(6) * Ax * zdx + T(12) * Bx * zdx2) / (dd * dd);
+  const T d2ay = (T(2) + T(6) * Ay * zdy + T(12) * By * zdy2) / (dd * dd);
+  w.d2sx = -(sx0 / (T(4) * (ax * sqax))) * (dax * dax) + hx * d2ax;
+  w.d2sy = -(sy0 / (T(4) * (ay * sqay))) * (day * day) + hy * d2ay;
+}
+
+template <int MODEL, class T>
+__device__ __forceinline__ void model_widths(const T* th, const float* psf, Widths<T>& w) {
+  w.dsx = w.dsy = w.d2sx = w.d2sy = T(0);
+  if constexpr (MODEL == XYNB) {
+    w.sx = w.sy = (T)psf[0];
+  } else if constexpr (MODEL == XYNBS) {
+    w.sx = w.sy = th[4];
+  } else if constexpr (MODEL == XYNBSXSY) {
+    w.sx = th[4];
+    w.sy = th[5];
+  } else {
+    astig_widths<T>(th[2], psf, w);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// One Newton iteration's gradient G[P] and diagonal Hessian H[P] over this
+// lane's column block (unified_kernel.jl:319-334):
+//   G_k = sum_ij dudt_k cf,   H_k = sum_ij (d2udt2_k cf - dudt_k^2 df).
+// Each per-pixel term factors as (column table)(row table)(cf or df), so the
+// pixel loop only accumulates row-weighted cf / df sums per column
+//   S1 = sum_i phiy cf    S2 = sum_i phiy^2 df   S3 = sum_i Dy cf
+//   S4 = sum_i D2y cf     S5 = sum_i Dy^2 df     S6 = sum_i cf    S7 = sum_i df
+//   S8 = sum_i Ay cf      S9 = sum_i AAy cf      S10 = sum_i phiy Ay df
+//   S11 = sum_i Ay^2 df
+// (A = derivative w.r.t. the sigma / z parameter on the row axis, AA its second
+// derivative) that are then multiplied by the column's tables, which are built
+// from the column's two edges on the fly.
+// FAST (Poisson only): every model value exceeds 0.01 and neither 1e4 cap
+// (camera_models.jl:23-28) can bind, so cf + 1 = data/model is accumulated and
+// the constant row sums (sphi, sD, ...) are subtracted once per column.
+// ---------------------------------------------------------------------------
+template <int MODEL, int CAM, int N, int L, bool FAST>
+__device__ __forceinline__ void accumulate(const float* dv, const float* vr, int c0, float nph, float bg,
+                                           const AxisTab<float, N>& ry, const AxisK<float>& kc,
+                                           const Widths<float>& w, const float* Ay, const float* AAy,
+                                           float sphi, float sD, float sD2, float sA, float sAA, float* G,
+                                           float* H) {
+  using Tr = MT<MODEL>;
+  constexpr int P = Tr::P;
+  constexpr int CB = (N + L - 1) / L;
+  constexpr bool SIG = Tr::EXTRA != 0;
+  constexpr bool TWO = (Tr::EXTRA == 1 || Tr::EXTRA == 3);
+#pragma unroll
+  for (int k = 0; k < P; ++k) { G[k] = 0.f; H[k] = 0.f; }
+  Edge<float> lo = edge_eval<float, SIG>(kc, c0);
+#pragma unroll
+  for (int cc = 0; cc < CB; ++cc) {
+    const Edge<float> hi = edge_eval<float, SIG>(kc, c0 + cc + 1);
+    const Pix<float> cx = pix_from_edges<float, SIG>(kc, lo, hi);
+    lo = hi;
+    const bool cvalid = (L == 1) || (c0 + cc < N);
+    const float nphx = nph * cx.phi;
+    float S1 = 0.f, S2 = 0.f, S3 = 0.f, S4 = 0.f, S5 = 0.f, S6 = 0.f, S7 = 0.f;
+    float S8 = 0.f, S9 = 0.f, S10 = 0.f, S11 = 0.f;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const float d = dv[cc * N + i];
+      const float mu = fmaf(nphx, ry.phi[i], bg);
+      float cf, df;
+      if constexpr (CAM == IDEAL) {
+        const float rc = fast_rcp(mu);
+        const float q = d * rc;
+        if constexpr (FAST) {
+          cf = q;               // cf + 1; the constant is removed after the row sums
+          df = q * rc;
+        } else {
+          const bool ok = mu > 0.01f;   // camera_models.jl:23-32
+          cf = ok ? jmin(q - 1.f, 1.0e4f) : 0.f;
+          df = ok ? jmin(q * rc, 1.0e4f) : 0.f;
+        }
+      } else {
+        const float v = mu + vr[cc * N + i];   // camera_models.jl:36-42
+        const float rc = fast_rcp(v);
+        cf = (d - mu) * rc;
+        df = rc;
+      }
+      const float py = ry.phi[i];
+      S1 = fmaf(py, cf, S1);
+      S2 = fmaf(py * py, df, S2);
+      S3 = fmaf(ry.D[i], cf, S3);
+      S4 = fmaf(ry.D2[i], cf, S4);
+      S5 = fmaf(ry.D[i] * ry.D[i], df, S5);
+      S6 += cf;
+      S7 += df;
+      if constexpr (SIG) {
+        S8 = fmaf(Ay[i], cf, S8);
+        S9 = fmaf(AAy[i], cf, S9);
+        S11 = fmaf(Ay[i] * Ay[i], df, S11);
+        if constexpr (TWO) S10 = fmaf(py * Ay[i], df, S10);
+      }
+    }
+    if constexpr (CAM == IDEAL && FAST) {
+      S1 -= sphi;
+      S3 -= sD;
+      S4 -= sD2;
+      S6 -= (float)N;
+      if constexpr (SIG) { S8 -= sA; S9 -= sAA; }
+    }
+    if (!cvalid) {
+      S1 = S2 = S3 = S4 = S5 = S6 = S7 = S8 = S9 = S10 = S11 = 0.f;
+    }
+    const float px = cx.phi, Dx = cx.D, D2x = cx.D2;
+    G[Tr::IX] += Dx * S1;
+    H[Tr::IX] += D2x * S1 - Dx * Dx * S2;
+    G[Tr::IY] += px * S3;
+    H[Tr::IY] += px * S4 - px * px * S5;
+    G[Tr::IN] += px * S1;
+    H[Tr::IN] -= px * px * S2;
+    G[Tr::IBG] += S6;
+    H[Tr::IBG] -= S7;
+    if constexpr (SIG) {
+      float ax, aax;
+      if constexpr (MODEL == ASTIG) {
+        ax = w.dsx * cx.S;
+        aax = cx.SS * (w.dsx * w.dsx) + cx.S * w.d2sx;
+      } else {
+        ax = cx.S;
+        aax = cx.SS;
+      }
+      if constexpr (TWO) {
+        G[Tr::IE1] += ax * S1 + px * S8;
+        H[Tr::IE1] += aax * S1 + px * S9 - ax * ax * S2 - 2.f * ax * px * S10 - px * px * S11;
+      } else {
+        G[Tr::IE1] += ax * S1;
+        H[Tr::IE1] += aax * S1 - ax * ax * S2;
+        G[Tr::IE2] += px * S8;
+        H[Tr::IE2] += px * S9 - px * px * S11;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The kernel.
+// ---------------------------------------------------------------------------
+template <int MODEL, int CAM, int N, int L, int PREC>
+__global__ __launch_bounds__(kWave) void fit_kernel(FitArgs a) {
+  using Tr = MT<MODEL>;
+  constexpr int P = Tr::P;
+  constexpr int CB = (N + L - 1) / L;   // columns per lane
+  constexpr int RPW = kWave / L;        // ROIs per wave
+  constexpr bool SIG = Tr::EXTRA != 0;  // sigma / z terms present
+
+  const int lane = threadIdx.x;
+  const int r = (L == 1) ? 0 : (lane & (L - 1));
+  const int64_t roi_raw = (int64_t)blockIdx.x * RPW + (lane / L);
+  const bool active = roi_raw < a.n_rois;
+  const int64_t roi = active ? roi_raw : (a.n_rois - 1);
+  const int c0 = r * CB;
+
+  // ---- load this lane's column block (and variances) once ----------------
+  float dv[CB * N];
+  float vr[CAM == SCMOS ? CB * N : 1];
+  const float* src = a.data + roi * (int64_t)(N * N);
+  float dmax_l = 0.f;
+  int cxc = 1, cyc = 1;
+  bool inb = true;   // sCMOS ROI inside the variance map
+  if constexpr (CAM == SCMOS) {
+    cxc = a.x_corners[roi];
+    cyc = a.y_corners[roi];
+    inb = (cxc >= 1) && (cyc >= 1) && (cxc <= a.nx - N + 1) && (cyc <= a.ny - N + 1);
+  }
+#pragma unroll
+  for (int cc = 0; cc < CB; ++cc) {
+    const int col = c0 + cc;
+    const bool valid = (L == 1) || (col < N);
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const float v = valid ? src[col * N + i] : 0.f;
+      dv[cc * N + i] = v;
+      dmax_l = fmaxf(dmax_l, v);
+      if constexpr (CAM == SCMOS) {
+        const int64_t off = (int64_t)(cyc - 1 + i) + (int64_t)a.ny * (int64_t)(cxc - 1 + col);
+        vr[cc * N + i] = (valid && inb) ? a.var_map[off] : 0.f;
+      }
+    }
+  }
+  const float dmax = group_max<L>(dmax_l);
+
+  // ---- initial estimates (unified_kernel.jl:158-255) ---------------------
+  float th[P];
+  {
+    float esum = 0.f;
+#pragma unroll
+    for (int cc = 0; cc < CB; ++cc) {
+      const int col = c0 + cc;
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const bool edge = (i == 0) || (i == N - 1) || (col == 0) || (col == N - 1);
+        if (edge) esum += dv[cc * N + i];   // invalid columns hold zeros
+      }
+    }
+    esum = group_sum<L>(esum);
+    const float bg0 = esum / (float)(4 * N - 4);
+    float tot = 0.f, xw = 0.f, yw = 0.f;
+#pragma unroll
+    for (int cc = 0; cc < CB; ++cc) {
+      const bool valid = (L == 1) || (c0 + cc < N);
+      const float fj = (float)(c0 + cc + 1);
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const float s = valid ? fmaxf(0.f, dv[cc * N + i] - bg0) : 0.f;
+        tot += s;
+        xw += s * fj;
+        yw += s * (float)(i + 1);
+      }
+    }
+    tot = group_sum<L>(tot);
+    xw = group_sum<L>(xw);
+    yw = group_sum<L>(yw);
+    const float center = (float)(N + 1) * 0.5f;
+    float x0 = tot > 10.f ? xw / tot : center;
+    float y0 = tot > 10.f ? yw / tot : center;
+    x0 = jclamp(x0, 1.5f, (float)N - 0.5f);
+    y0 = jclamp(y0, 1.5f, (float)N - 0.5f);
+    const float ph = fmaxf(tot, 100.f);
+    th[Tr::IX] = x0;
+    th[Tr::IY] = y0;
+    th[Tr::IN] = ph;
+    th[Tr::IBG] = bg0;
+    if constexpr (MODEL == XYNBS) th[4] = 1.3f;
+    if constexpr (MODEL == XYNBSXSY) { th[4] = 1.3f; th[5] = 1.3f; }
+    if constexpr (MODEL == ASTIG) {
+      float mxx = 0.f, myy = 0.f;
+#pragma unroll
+      for (int cc = 0; cc < CB; ++cc) {
+        const bool valid = (L == 1) || (c0 + cc < N);
+        const float dxp = (float)(c0 + cc + 1) - x0;
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          const float s = valid ? fmaxf(0.f, dv[cc * N + i] - bg0) : 0.f;
+          const float dyp = (float)(i + 1) - y0;
+          mxx += s * dxp * dxp;
+          myy += s * dyp * dyp;
+        }
+      }
+      mxx = group_sum<L>(mxx);
+      myy = group_sum<L>(myy);
+      float z0 = 0.f;
+      if (tot > 10.f) {
+        const float sxe = sqrtf(fmaxf(0.5f, mxx / tot));
+        const float sye = sqrtf(fmaxf(0.5f, myy / tot));
+        const float rx = sxe / a.psf[0], ry = sye / a.psf[1];
+        const float dal = rx * rx - ry * ry;
+        z0 = -dal * (a.psf[7] * a.psf[7]) / (4.f * a.psf[6]);
+        z0 = jclamp(z0, -600.f, 600.f);
+      }
+      th[2] = z0;
+    }
+  }
+
+  // ---- Newton-Raphson iterations (unified_kernel.jl:314-351) -------------
+  for (int it = 0; it < a.iterations; ++it) {
+    Widths<float> w;
+    model_widths<MODEL, float>(th, a.psf, w);
+    const float nph = th[Tr::IN];
+    const float bg = th[Tr::IBG];
+
+    AxisTab<float, N> ry;
+    axis_tables<float, N, SIG>(th[Tr::IY], w.sy, nph, 0, ry);
+    const AxisK<float> kc = axis_consts<float>(th[Tr::IX], w.sx, nph);
+
+    // Row extras: A = d(model)/d(extra) on the row axis, AA its second derivative.
+    float Ay[SIG ? N : 1], AAy[SIG ? N : 1];
+    if constexpr (SIG) {
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        if constexpr (MODEL == ASTIG) {
+          Ay[i] = w.dsy * ry.S[i];
+          AAy[i] = ry.SS[i] * (w.dsy * w.dsy) + ry.S[i] * w.d2sy;
+        } else {
+          Ay[i] = ry.S[i];
+          AAy[i] = ry.SS[i];
+        }
+      }
+    }
+
+    // Poisson fast path: every pixel has model > 0.01 and no cap can bind.
+    // model >= bg - N*tiny for N <= 1e5 (phi >= 0 up to erf rounding), so
+    // bg >= 0.05 keeps it above 0.01, and dmax <= 5000 bg min(bg,1) keeps
+    // data/model - 1 and data/model^2 below 1e4.
+    bool fast = false;
+    if constexpr (CAM == IDEAL) {
+      bool fin = true;
+#pragma unroll
+      for (int k = 0; k < P; ++k) fin = fin && (fabsf(th[k]) < 3.0e38f);
+      fast = fin && (bg >= 0.05f) && (nph >= 0.f) && (nph <= 1.0e5f) &&
+             (dmax <= 5000.f * bg * fminf(bg, 1.f));
+    }
+
+    // Row sums for the q = cf + 1 form of the fast path.
+    float sphi = 0.f, sD = 0.f, sD2 = 0.f, sA = 0.f, sAA = 0.f;
+    if (fast) {
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        sphi += ry.phi[i];
+        sD += ry.D[i];
+        sD2 += ry.D2[i];
+        if constexpr (SIG) { sA += Ay[i]; sAA += AAy[i]; }
+      }
+    }
+
+    float G[P], H[P];
+    if (fast) {
+      accumulate<MODEL, CAM, N, L, true>(dv, vr, c0, nph, bg, ry, kc, w, Ay, AAy, sphi, sD, sD2, sA, sAA, G, H);
+    } else {
+      accumulate<MODEL, CAM, N, L, false>(dv, vr, c0, nph, bg, ry, kc, w, Ay, AAy, sphi, sD, sD2, sA, sAA, G, H);
+    }
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      G[k] = group_sum<L>(G[k]);
+      H[k] = group_sum<L>(H[k]);
+    }
+    // Scalar Newton step with constraints (unified_kernel.jl:337-350, constraints.jl:172-177).
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      const float step = fabsf(H[k]) > 1e-10f ? G[k] / H[k] : 0.f;
+      const float ms = a.max_step[k];
+      const float t = th[k] - jclamp(step, -ms, ms);
+      th[k] = jclamp(t, a.lower[k], a.upper[k]);
+    }
+  }
+
+  // ---- final pass: LLR + Fisher + CRLB (unified_kernel.jl:353-433) --------
+  using TF = typename std::conditional<PREC == PREC_MIXED, double, float>::type;
+  TF thf[P];
+#pragma unroll
+  for (int k = 0; k < P; ++k) thf[k] = (TF)th[k];
+  Widths<TF> wf;
+  model_widths<MODEL, TF>(thf, a.psf, wf);
+  const TF nphf = thf[Tr::IN], bgf = thf[Tr::IBG];
+  AxisTab<TF, N> fy;
+  axis_tables<TF, N, SIG>(thf[Tr::IY], wf.sy, nphf, 0, fy);
+  const AxisK<TF> kf = axis_consts<TF>(thf[Tr::IX], wf.sx, nphf);
+
+  constexpr int NF = P * (P + 1) / 2;
+  TF Fu[NF];
+#pragma unroll
+  for (int q = 0; q < NF; ++q) Fu[q] = TF(0);
+  TF llr = TF(0);
+  Edge<TF> flo = edge_eval<TF, false>(kf, c0);
+#pragma unroll
+  for (int cc = 0; cc < CB; ++cc) {
+    const Edge<TF> fhi = edge_eval<TF, false>(kf, c0 + cc + 1);
+    const Pix<TF> fx = pix_from_edges<TF, SIG>(kf, flo, fhi);
+    flo = fhi;
+    const TF px = fx.phi;
+    const bool cvalid = (L == 1) || (c0 + cc < N);
+    TF ax = TF(0);
+    if constexpr (SIG) ax = (MODEL == ASTIG) ? wf.dsx * fx.S : fx.S;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const TF py = fy.phi[i];
+      const TF d = (TF)dv[cc * N + i];
+      const TF mu = bgf + nphf * px * py;
+      TF du[P];
+      du[Tr::IX] = fx.D * py;
+      du[Tr::IY] = fy.D[i] * px;
+      du[Tr::IN] = px * py;
+      du[Tr::IBG] = TF(1);
+      if constexpr (SIG) {
+        const TF ay = (MODEL == ASTIG) ? wf.dsy * fy.S[i] : fy.S[i];
+        if constexpr (Tr::EXTRA == 2) {
+          du[Tr::IE1] = ax * py;
+          du[Tr::IE2] = ay * px;
+        } else {
+          du[Tr::IE1] = ax * py + ay * px;
+        }
+      }
+      TF v = mu;
+      TF lp;
+      if constexpr (CAM == IDEAL) {
+        // camera_models.jl:60-71
+        if constexpr (PREC == PREC_MIXED) {
+          lp = (mu > TF(0) && d > TF(0)) ? d * log(mu / d) + (d - mu) : (mu > TF(0) && d == TF(0)) ? -mu : TF(0);
+        } else {
+          lp = (mu > TF(0) && d > TF(0)) ? d * logf(mu) - mu - (d * logf(d) - d)
+                                        : (mu > TF(0) && d == TF(0)) ? -mu : TF(0);
+        }
+      } else {
+        // camera_models.jl:73-85
+        v = mu + (TF)vr[cc * N + i];
+        const TF res = d - mu;
+        lp = v > TF(0) ? TF(-0.5) * (res * res / v) : TF(0);
+      }
+      if (cvalid) llr += lp;
+      if (cvalid && mu > TF(0)) {   // unified_kernel.jl:378-396
+        const TF wv = TF(1) / v;
+        int q = 0;
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+          const TF ak = du[k] * wv;
+#pragma unroll
+          for (int l = k; l < P; ++l) Fu[q++] += ak * du[l];
+        }
+      }
+    }
+  }
+  llr = group_sum<L>(llr);
+#pragma unroll
+  for (int q = 0; q < NF; ++q) Fu[q] = group_sum<L>(Fu[q]);
+  TF F[P * P];
+  {
+    int q = 0;
+#pragma unroll
+    for (int k = 0; k < P; ++k)
+#pragma unroll
+      for (int l = k; l < P; ++l) {
+        F[k + P * l] = Fu[q];
+        F[l + P * k] = Fu[q];
+        ++q;
+      }
+  }
+  TF crlb[P];
+  invert_fisher<P, TF>(F, crlb);
+
+  if (!inb) {   // out-of-range sCMOS corner: no fit, NaN outputs
+    const float qnan = __builtin_nanf("");
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      th[k] = qnan;
+      crlb[k] = TF(qnan);
+    }
+    llr = TF(qnan);
+  }
+  if (active && r == 0) {
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      a.params[roi * P + k] = th[k];
+      a.crlb[roi * P + k] = (float)crlb[k];
+    }
+    a.llr[roi] = (float)llr;
+    if constexpr (PREC == PREC_MIXED) {
+      if (a.crlb64) {
+#pragma unroll
+        for (int k = 0; k < P; ++k) a.crlb64[roi * P + k] = crlb[k];
+      }
+      if (a.llr64) a.llr64[roi] = llr;
+    }
+  }
+}
+
+template <int MODEL, int CAM, int N, int L, int PREC>
+hipError_t launch_fit(const FitArgs& a, hipStream_t stream) {
+  constexpr int RPW = kWave / L;
+  const int64_t blocks = (a.n_rois + RPW - 1) / RPW;
+  if (blocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL((fit_kernel<MODEL, CAM, N, L, PREC>), dim3((unsigned)blocks), dim3(kWave), 0, stream, a);
+  return hipGetLastError();
+}
+
+}  // namespace gmle

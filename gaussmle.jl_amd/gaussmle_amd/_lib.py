@@ -1,0 +1,130 @@
+"""ctypes binding of libgaussmle_hip.so (the C ABI in include/gaussmle.h).
+
+The product path has no CPU fallback: if the HIP library is missing, loading it
+raises immediately.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG_ROOT = Path(__file__).resolve().parent.parent
+LIB_PATH = PKG_ROOT / "lib" / "libgaussmle_hip.so"
+
+GMLE_OK = 0
+GMLE_ERR_ARG = -1
+GMLE_ERR_BOUNDS = -2
+GMLE_ERR_DATA = -3
+GMLE_ERR_HIP = -4
+GMLE_ERR_NOMEM = -5
+GMLE_ERR_NODEVICE = -6
+GMLE_ERR_UNSUPPORTED = -7
+
+MODEL_IDS = {"GaussianXYNB": 0, "GaussianXYNBS": 1, "GaussianXYNBSXSY": 2, "AstigmaticXYZNB": 3}
+CAMERA_IDS = {"ideal": 0, "scmos": 1}
+PRECISION_IDS = {"fp32": 0, "mixed": 1}
+
+# Every function the header declares, with its ctypes signature.
+_F = C.POINTER(C.c_float)
+_D = C.POINTER(C.c_double)
+_I32 = C.POINTER(C.c_int32)
+SIGNATURES = {
+    "gmle_version": (C.c_char_p, []),
+    "gmle_device_count": (C.c_int, []),
+    "gmle_last_error": (C.c_size_t, [C.c_char_p, C.c_size_t]),
+    "gmle_n_params": (C.c_int, [C.c_int]),
+    "gmle_default_constraints": (C.c_int, [C.c_int, C.c_int, _F, _F, _F]),
+    "gmle_max_box": (C.c_int, []),
+    "gmle_lanes_supported": (C.c_int, [C.c_int, C.c_int]),
+    "gmle_default_lanes": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int64]),
+    "gmle_fit": (
+        C.c_int,
+        [C.c_int, _F, C.c_int, C.c_int64, _F, C.c_int, _F, C.c_int, C.c_int, _I32, _I32,
+         _F, _F, _F, C.c_int, C.c_int, C.c_int, _F, _F, _F, _D, _D],
+    ),
+    "gmle_fit_device": (
+        C.c_int,
+        [C.c_int, _F, C.c_int, C.c_int64, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
+         C.c_void_p, _F, _F, _F, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+         C.c_void_p, C.c_void_p],
+    ),
+}
+
+
+class GaussMLEError(RuntimeError):
+    """Raised when the HIP library returns an error code."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"[{code}] {message}")
+        self.code = code
+
+
+class ArgumentError(GaussMLEError, ValueError):
+    """The reference's ArgumentError (invalid input rejected before launch)."""
+
+
+_lib = None
+
+
+def load(path: str | os.PathLike | None = None) -> C.CDLL:
+    """Load (once) and return the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path is not None else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(
+            f"HIP library not found at {p}; build it with `python gaussmle.jl_amd/build.py` "
+            "(there is no CPU fallback)")
+    lib = C.CDLL(str(p))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def last_error() -> str:
+    lib = load()
+    buf = C.create_string_buffer(1024)
+    lib.gmle_last_error(buf, len(buf))
+    return buf.value.decode(errors="replace")
+
+
+def check(rc: int) -> None:
+    if rc == GMLE_OK:
+        return
+    msg = last_error()
+    if rc in (GMLE_ERR_ARG, GMLE_ERR_BOUNDS, GMLE_ERR_DATA):
+        raise ArgumentError(rc, msg)
+    raise GaussMLEError(rc, msg)
+
+
+def fptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_F)
+
+
+def dptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_D)
+
+
+def iptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_I32)
+
+
+def device_count() -> int:
+    return int(load().gmle_device_count())

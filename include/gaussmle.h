@@ -1,0 +1,121 @@
+/*
+ * gaussmle.h -- C ABI of libgaussmle_hip.so, the gfx950 (MI355X) drop-in for
+ * GaussMLE.jl's per-ROI maximum-likelihood Gaussian fit.
+ *
+ * The reference launches one KernelAbstractions kernel per batch,
+ *   unified_gaussian_mle_kernel!(backend)(results, uncertainties, log_likelihoods,
+ *       data, psf_pixels, Val(scmos), variance_map, x_corners, y_corners,
+ *       constraints, iterations; ndrange = n)        (src/unified_kernel.jl:285-297)
+ * from six call sites in src/interface.jl (:211-216, :249-256, :328-333,
+ * :358-364, :415-420, :445-451).  gmle_fit() replaces one such launch together
+ * with the GPU branch's batch loop (src/interface.jl:220-262, :336-369,
+ * :423-456): same arrays, same layout, same outputs.
+ *
+ * Conventions (all follow the reference's Julia arrays, passed without copies):
+ *   data         Float32 (box, box, n_rois), column-major; ROI k starts at
+ *                box*box*k; pixel (row i, column j) at (i-1) + box*(j-1).
+ *   params, crlb Float32 (P, n_rois); ROI k's P values are contiguous.
+ *   llr          Float32 (n_rois), log-likelihood ratio at the estimate.
+ *   var_map      Float32 (ny, nx), column-major, indexed [row = y, col = x]
+ *                with row = y_corner + i - 1, col = x_corner + j - 1.
+ *   x/y_corners  Int32, 1-based camera coordinates of each ROI's first pixel.
+ *   psf          8 floats in pixel units (after to_pixel_units,
+ *                src/psf_models.jl:311-331):
+ *                  GMLE_XYNB  : psf[0] = sigma
+ *                  GMLE_ASTIG : sigma_x0, sigma_y0, Ax, Ay, Bx, By, gamma, d
+ *                  others     : ignored (may be NULL)
+ *   lower, upper, max_step   P floats each (ParameterConstraints{P},
+ *                src/constraints.jl:6-54).
+ * Parameter order: XYNB [x, y, N, bg]; XYNBS [x, y, N, bg, sigma];
+ * XYNBSXSY [x, y, N, bg, sigma_x, sigma_y]; ASTIG [x, y, z, N, bg].
+ *
+ * Errors: every entry point returns GMLE_OK (0) or a negative code and never
+ * throws.  gmle_last_error() returns the calling thread's last message.
+ * Numerical failure is in-band, exactly as in the reference: a singular Fisher
+ * matrix gives CRLB = +Inf with the estimate still written
+ * (src/unified_kernel.jl:425-430).
+ */
+#ifndef GAUSSMLE_H
+#define GAUSSMLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GMLE_OK 0
+#define GMLE_ERR_ARG (-1)         /* invalid argument (the reference's ArgumentError) */
+#define GMLE_ERR_BOUNDS (-2)      /* sCMOS ROI outside the variance map */
+#define GMLE_ERR_DATA (-3)        /* NaN or Inf in the input data */
+#define GMLE_ERR_HIP (-4)         /* HIP runtime error */
+#define GMLE_ERR_NOMEM (-5)       /* device allocation failed */
+#define GMLE_ERR_NODEVICE (-6)    /* no usable GPU */
+#define GMLE_ERR_UNSUPPORTED (-7) /* box size / lanes-per-ROI not built */
+
+enum gmle_model { GMLE_XYNB = 0, GMLE_XYNBS = 1, GMLE_XYNBSXSY = 2, GMLE_ASTIG = 3 };
+enum gmle_camera { GMLE_IDEAL = 0, GMLE_SCMOS = 1 };
+enum gmle_precision {
+  GMLE_FP32 = 0,  /* everything Float32, as the reference */
+  GMLE_MIXED = 1  /* Float32 Newton iterations; Float64 LLR, Fisher, inversion, CRLB */
+};
+
+/* Library version string. */
+const char* gmle_version(void);
+
+/* Number of usable GPUs; 0 when none.  Replaces the CUDA.functional() probe of
+ * src/devices.jl:11-35 and src/interface.jl:123-137. */
+int gmle_device_count(void);
+
+/* Copies the calling thread's last error message into buf (NUL-terminated,
+ * truncated to len); returns the full message length. */
+size_t gmle_last_error(char* buf, size_t len);
+
+/* Number of fitted parameters of a model (length(psf_model), src/psf_models.jl:160-161),
+ * or GMLE_ERR_ARG. */
+int gmle_n_params(int model);
+
+/* default_constraints(psf_model, box_size) (src/constraints.jl:16-47): fills P
+ * floats of each array. */
+int gmle_default_constraints(int model, int box, float* lower, float* upper, float* max_step);
+
+/* Largest box size supported by the kernels (smallest is 3). */
+int gmle_max_box(void);
+
+/* 1 when a kernel with this many lanes per ROI exists for the box size. */
+int gmle_lanes_supported(int box, int lanes);
+
+/* Lanes per ROI chosen when a fit call passes lanes = 0: the library's tuned
+ * choice for this shape and batch size. */
+int gmle_default_lanes(int model, int camera, int box, int precision, int64_t n_rois);
+
+/* Replaces one reference kernel launch plus the GPU batch loop
+ * (src/unified_kernel.jl:285-297; src/interface.jl:220-262).  All pointers are
+ * host memory; the call is synchronous and keeps no pointer after it returns.
+ * Validates like validate_fit_input (src/interface.jl:10-40): empty input,
+ * box < 3 and NaN/Inf data are errors; sCMOS ROIs must lie inside var_map
+ * (the reference reads it unchecked).  var_map / corners may be NULL for
+ * GMLE_IDEAL.  crlb64 / llr64 (optional, GMLE_MIXED only) receive the Float64
+ * CRLB (P, n_rois) and LLR (n_rois).  lanes = 0 selects automatically. */
+int gmle_fit(int model, const float* psf, int box, int64_t n_rois, const float* data, int camera,
+             const float* var_map, int ny, int nx, const int32_t* x_corners, const int32_t* y_corners,
+             const float* lower, const float* upper, const float* max_step, int iterations, int precision,
+             int lanes, float* params, float* crlb, float* llr, double* crlb64, double* llr64);
+
+/* Same computation with every array already in device memory (HBM), enqueued
+ * on `stream` (a hipStream_t; NULL = the null stream) and returning without
+ * waiting.  psf / lower / upper / max_step are host arrays.  Inputs are not
+ * scanned for NaN; an sCMOS ROI whose corner lies outside var_map yields NaN
+ * outputs for that ROI instead of an out-of-bounds read. */
+int gmle_fit_device(int model, const float* psf, int box, int64_t n_rois, const float* d_data, int camera,
+                    const float* d_var_map, int ny, int nx, const int32_t* d_x_corners,
+                    const int32_t* d_y_corners, const float* lower, const float* upper, const float* max_step,
+                    int iterations, int precision, int lanes, float* d_params, float* d_crlb, float* d_llr,
+                    double* d_crlb64, double* d_llr64, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GAUSSMLE_H */

@@ -1,0 +1,391 @@
+"""Pins for the CPU oracle (oracle/refcpu.*) against independent mathematics.
+
+GaussMLE.jl ships no golden vectors and cannot run here (Julia is absent), so
+every oracle function is checked against something it does not share code
+with: quadrature, scipy's erf and distributions, finite differences, numpy's
+matrix inverse, and an independent numpy/scipy maximum-likelihood fit.  Each
+check is chosen so that a plausible slip (a dropped term, a sign, swapped x/y,
+a wrong pixel edge) fails it.  The statistical checks reuse the reference's own
+tolerances (test/validation_utils.jl:340-350, test/model_validation_tests.jl).
+"""
+import math
+
+import mpmath
+import numpy as np
+import pytest
+from scipy import optimize, special, stats
+
+XYNB, XYNBS, XYNBSXSY, ASTIG = 0, 1, 2, 3
+ASTIG_PSF = [1.3, 1.3, 0.05, 0.05, 0.3, 0.3, 50.0, 100.0]
+
+
+# ---------------------------------------------------------------------------
+# Pixel-integrated Gaussian (src/gausslib/baselibrary.jl:8-14)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("i,pos,sigma", [(1, 1.0, 1.3), (5, 6.2, 0.9), (3, 9.7, 2.1), (11, 6.0, 1.3),
+                                         (1, 7.5, 0.4), (7, 7.25, 4.0)])
+def test_integral_matches_quadrature(oracle, i, pos, sigma):
+    mpmath.mp.dps = 30
+    pdf = lambda t: mpmath.exp(-(t - pos) ** 2 / (2 * sigma ** 2)) / (sigma * mpmath.sqrt(2 * mpmath.pi))
+    exact = float(mpmath.quad(pdf, [i - 0.5, i + 0.5]))
+    assert oracle.integral_gaussian_1d(i, pos, sigma, oracle.FP64) == pytest.approx(exact, abs=1e-14, rel=1e-12)
+    assert oracle.integral_gaussian_1d(i, pos, sigma, oracle.FP32) == pytest.approx(exact, abs=2e-7, rel=2e-6)
+
+
+def test_integral_sums_to_one(oracle):
+    tot = sum(oracle.integral_gaussian_1d(k, 20.3, 1.7, oracle.FP64) for k in range(1, 41))
+    assert tot == pytest.approx(1.0, abs=1e-12)
+    # the pixel centred on the mean holds erf(1/(2 sqrt 2 sigma))
+    assert oracle.integral_gaussian_1d(10, 10.0, 1.3, oracle.FP64) == pytest.approx(
+        special.erf(0.5 / (math.sqrt(2) * 1.3)), rel=1e-14)
+
+
+# ---------------------------------------------------------------------------
+# Model derivatives (src/psf_derivatives.jl) by central finite differences
+# ---------------------------------------------------------------------------
+THETAS = {
+    XYNB: ([5.3, 6.7, 1500.0, 8.0], [1.3]),
+    XYNBS: ([5.3, 6.7, 1500.0, 8.0, 1.45], None),
+    XYNBSXSY: ([5.3, 6.7, 1500.0, 8.0, 1.2, 1.6], None),
+    ASTIG: ([5.3, 6.7, -120.0, 1500.0, 8.0], ASTIG_PSF),
+}
+
+
+def _fd(f, th, k, h):
+    a, b = np.array(th, float), np.array(th, float)
+    a[k] += h
+    b[k] -= h
+    return (f(a) - f(b)) / (2 * h)
+
+
+def _phi(k, mu, s):
+    t = math.sqrt(0.5) / s
+    return 0.5 * (special.erf((k - mu + 0.5) * t) - special.erf((k - mu - 0.5) * t))
+
+
+@pytest.mark.parametrize("model", [XYNB, XYNBS, XYNBSXSY, ASTIG])
+@pytest.mark.parametrize("pix", [(6, 5), (2, 9), (7, 7), (11, 1)])
+def test_first_derivatives(oracle, model, pix):
+    th, psf = THETAS[model]
+    i, j = pix
+    P = len(th)
+    m, du, d2 = oracle.pixel_derivs(model, psf, i, j, th, oracle.FP64)
+    for k in range(P):
+        h = 1e-3 if (model == ASTIG and k == 2) else 1e-5
+        fd = _fd(lambda t: oracle.pixel_derivs(model, psf, i, j, t, oracle.FP64)[0], th, k, h)
+        assert du[k] == pytest.approx(fd, rel=2e-6, abs=1e-9 * max(1.0, abs(m))), (k, du[k], fd)
+
+
+@pytest.mark.parametrize("model", [XYNB, XYNBS, XYNBSXSY, ASTIG])
+@pytest.mark.parametrize("pix", [(6, 5), (2, 9), (7, 7)])
+def test_diagonal_second_derivatives(oracle, model, pix):
+    """d2udt2 is the exact diagonal for x, y and a single sigma; the reference
+    omits the sigma_x-sigma_y cross term for XYNBS's shared sigma and for z
+    (src/gausslib/baselibrary.jl:61-67, src/psf_derivatives.jl:155-163)."""
+    th, psf = THETAS[model]
+    i, j = pix
+    m, du, d2 = oracle.pixel_derivs(model, psf, i, j, th, oracle.FP64)
+    P = len(th)
+    for k in range(P):
+        h = 1e-3 if (model == ASTIG and k == 2) else 1e-5
+        fd = _fd(lambda t: oracle.pixel_derivs(model, psf, i, j, t, oracle.FP64)[1][k], th, k, h)
+        expect = fd
+        N = th[3] if model == ASTIG else th[2]
+        if model == XYNBS and k == 4:
+            s = th[4]
+            dpx = (_phi(j, th[0], s + 1e-6) - _phi(j, th[0], s - 1e-6)) / 2e-6
+            dpy = (_phi(i, th[1], s + 1e-6) - _phi(i, th[1], s - 1e-6)) / 2e-6
+            expect = fd - 2 * N * dpx * dpy
+        if model == ASTIG and k == 2:
+            sx0, sy0, Ax, Ay, Bx, By, g, d = psf
+            sig = lambda z, s0, A, B, sg: s0 * math.sqrt(1 + ((z + sg) / d) ** 2 + A * ((z + sg) / d) ** 3 +
+                                                       B * ((z + sg) / d) ** 4)
+            z = th[2]
+            sx, sy = sig(z, sx0, Ax, Bx, -g), sig(z, sy0, Ay, By, g)
+            dsx = (sig(z + 1e-4, sx0, Ax, Bx, -g) - sig(z - 1e-4, sx0, Ax, Bx, -g)) / 2e-4
+            dsy = (sig(z + 1e-4, sy0, Ay, By, g) - sig(z - 1e-4, sy0, Ay, By, g)) / 2e-4
+            dpx = (_phi(j, th[0], sx + 1e-6) - _phi(j, th[0], sx - 1e-6)) / 2e-6
+            dpy = (_phi(i, th[1], sy + 1e-6) - _phi(i, th[1], sy - 1e-6)) / 2e-6
+            expect = fd - 2 * N * dpx * dsx * dpy * dsy
+        assert d2[k] == pytest.approx(expect, rel=2e-5, abs=1e-7 * max(1.0, abs(du[k]))), (k, d2[k], expect)
+    # N and bg enter linearly
+    iN, ibg = (3, 4) if model == ASTIG else (2, 3)
+    assert d2[iN] == 0.0 and d2[ibg] == 0.0
+
+
+def test_model_value_layout(oracle):
+    """Pixel (row i, column j): x is fitted against the column, y against the row
+    (src/psf_derivatives.jl:10-15)."""
+    th = [3.0, 8.0, 1000.0, 2.0]
+    m_near, _, _ = oracle.pixel_derivs(XYNB, [1.3], 8, 3, th, oracle.FP64)   # row 8 = y, col 3 = x
+    m_far, _, _ = oracle.pixel_derivs(XYNB, [1.3], 3, 8, th, oracle.FP64)
+    s = float(np.float32(1.3))   # the PSF argument is Float32
+    assert m_near == pytest.approx(2.0 + 1000.0 * _phi(3, 3.0, s) * _phi(8, 8.0, s), rel=1e-13)
+    assert m_far < 2.1
+
+
+def test_fp32_derivatives_track_fp64(oracle):
+    for model, (th, psf) in THETAS.items():
+        for (i, j) in [(6, 5), (3, 8)]:
+            a = oracle.pixel_derivs(model, psf, i, j, th, oracle.FP32)
+            b = oracle.pixel_derivs(model, psf, i, j, th, oracle.FP64)
+            assert a[0] == pytest.approx(b[0], rel=1e-5)
+            np.testing.assert_allclose(a[1], b[1], rtol=2e-4, atol=1e-4 * np.max(np.abs(b[1])))
+
+
+# ---------------------------------------------------------------------------
+# Camera likelihood terms (src/camera_models.jl:19-96)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("d,mu", [(7.0, 3.0), (0.0, 2.5), (120.0, 131.7), (1.0, 0.02)])
+def test_poisson_terms_match_distribution(oracle, d, mu):
+    cf, df, ll = oracle.pixel_terms(0, d, mu, prec=oracle.FP64)
+    lp = lambda m: stats.poisson.logpmf(d, m)
+    h = 1e-6 * mu
+    assert cf == pytest.approx((lp(mu + h) - lp(mu - h)) / (2 * h), rel=1e-6, abs=1e-8)   # d/mu - 1
+    assert df == pytest.approx(d / mu ** 2, rel=1e-14)                                      # -E-free curvature
+    sat = stats.poisson.logpmf(d, d) if d > 0 else 0.0
+    assert ll == pytest.approx(lp(mu) - sat, rel=1e-10, abs=1e-10)                          # log-likelihood ratio
+
+
+def test_poisson_thresholds_and_caps(oracle):
+    assert oracle.pixel_terms(0, 5.0, 0.005, prec=oracle.FP64)[:2] == (0.0, 0.0)        # model <= 0.01
+    assert oracle.pixel_terms(0, 5.0, 0.01, prec=oracle.FP64)[:2] == (0.0, 0.0)
+    cf, df, _ = oracle.pixel_terms(0, 1e6, 1.0, prec=oracle.FP64)
+    assert (cf, df) == (1e4, 1e4)                                                        # both capped
+    cf, df, _ = oracle.pixel_terms(0, 1e3, 0.2, prec=oracle.FP64)
+    assert cf == pytest.approx(4999.0) and df == 1e4                                     # only df capped
+    assert oracle.pixel_terms(0, 0.0, 3.0, prec=oracle.FP64)[2] == -3.0                  # data == 0 -> -model
+    assert oracle.pixel_terms(0, 4.0, -1.0, prec=oracle.FP64)[2] == 0.0                  # model <= 0 -> 0
+
+
+@pytest.mark.parametrize("d,mu,v", [(10.0, 4.0, 2.0), (-3.0, 1.5, 0.7), (250.0, 240.0, 55.0)])
+def test_scmos_terms_match_gaussian(oracle, d, mu, v):
+    cf, df, ll = oracle.pixel_terms(1, d, mu, v, prec=oracle.FP64)
+    tv = mu + v
+    assert cf == pytest.approx((d - mu) / tv, rel=1e-14)
+    assert df == pytest.approx(1.0 / tv, rel=1e-14)
+    sd = math.sqrt(tv)
+    assert ll == pytest.approx(stats.norm.logpdf(d, mu, sd) - stats.norm.logpdf(d, d, sd), rel=1e-12)
+    assert oracle.pixel_terms(1, 3.0, -5.0, 1.0, prec=oracle.FP64)[2] == 0.0             # total variance <= 0
+
+
+# ---------------------------------------------------------------------------
+# Fisher inversion (src/unified_kernel.jl:7-137, 399-431)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("P", [4, 5, 6])
+def test_cholesky_inverse_matches_numpy(oracle, P):
+    rng = np.random.default_rng(P)
+    for _ in range(20):
+        A = rng.standard_normal((P, P))
+        A = A @ A.T + 0.1 * np.eye(P)
+        inv = oracle.cholesky_inverse(A, oracle.FP64)
+        np.testing.assert_allclose(inv, np.linalg.inv(A), rtol=1e-9, atol=1e-12 * np.abs(np.linalg.inv(A)).max())
+        inv32 = oracle.cholesky_inverse(A, oracle.FP32)
+        np.testing.assert_allclose(np.diag(inv32), np.diag(np.linalg.inv(A)), rtol=5e-3)
+
+
+@pytest.mark.parametrize("P", [4, 5, 6])
+def test_lu_inverse_matches_numpy(oracle, P):
+    rng = np.random.default_rng(10 + P)
+    for _ in range(20):
+        A = rng.standard_normal((P, P)) + P * np.eye(P)   # diagonally dominant: no pivoting needed
+        inv, ok = oracle.lu_inverse(A, oracle.FP64)
+        assert ok
+        np.testing.assert_allclose(inv, np.linalg.inv(A), rtol=1e-9, atol=1e-12)
+
+
+def test_inversion_paths(oracle):
+    rng = np.random.default_rng(3)
+    B = rng.standard_normal((5, 5))
+    spd = B @ B.T + np.eye(5)
+    crlb, path = oracle.invert_fisher(spd)
+    assert path == 0
+    np.testing.assert_allclose(crlb, np.sqrt(np.diag(np.linalg.inv(spd))), rtol=1e-8)
+    # symmetric indefinite: Cholesky fails, LU succeeds; negative variances clamp to 0
+    ind = np.diag([4.0, -2.0, 3.0, 1.0, 5.0])
+    crlb, path = oracle.invert_fisher(ind)
+    assert path == 1
+    np.testing.assert_allclose(crlb, [0.5, 0.0, 1 / math.sqrt(3), 1.0, 1 / math.sqrt(5)], rtol=1e-8)
+    # a zero row/column is rescued by the 1e-10 max|F| ridge (huge, finite CRLB)
+    sing = spd.copy()
+    sing[2, :] = 0
+    sing[:, 2] = 0
+    crlb, path = oracle.invert_fisher(sing)
+    assert path == 0 and crlb[2] == pytest.approx(1 / math.sqrt(1e-10 * np.abs(spd).max()), rel=1e-6)
+    # a pivot the ridge cancels exactly: Cholesky and LU both fail -> Inf
+    crlb, path = oracle.invert_fisher(np.diag([1.0, -1e-10, 1.0, 1.0]))
+    assert path == 2 and np.all(np.isinf(crlb))
+    # an all-zero matrix has a zero LU tolerance: the LU "succeeds" on 0/0 -> NaN,
+    # as Julia's max(0, NaN) propagates NaN
+    crlb, path = oracle.invert_fisher(np.zeros((4, 4)))
+    assert path == 1 and np.all(np.isnan(crlb))
+    # the 1e-10 max|F| ridge makes a tiny-but-positive pivot succeed
+    F = np.diag([1.0, 1e-12, 1.0, 1.0])
+    crlb, path = oracle.invert_fisher(F)
+    assert path == 0 and crlb[1] == pytest.approx(1 / math.sqrt(1e-12 + 1e-10), rel=1e-9)
+
+
+# ---------------------------------------------------------------------------
+# Initial estimates (src/unified_kernel.jl:158-255)
+# ---------------------------------------------------------------------------
+def test_initialization_moments(oracle):
+    rng = np.random.default_rng(5)
+    roi = rng.poisson(4.0, (11, 11)).astype(np.float32)
+    roi[3, 7] += 400   # row 4, col 8 (1-based)
+    th = oracle.initialize(XYNB, [1.3], roi, oracle.FP64)
+    border = np.concatenate([roi[0, :], roi[-1, :], roi[1:-1, 0], roi[1:-1, -1]]).astype(np.float64)
+    bg = border.mean()
+    s = np.maximum(0.0, roi.astype(np.float64) - bg)
+    rows, cols = np.mgrid[1:12, 1:12]
+    assert th[3] == pytest.approx(bg, rel=1e-12)
+    assert th[0] == pytest.approx((s * cols).sum() / s.sum(), rel=1e-12)   # x from columns
+    assert th[1] == pytest.approx((s * rows).sum() / s.sum(), rel=1e-12)   # y from rows
+    assert th[2] == pytest.approx(max(s.sum(), 100.0), rel=1e-12)
+    flat = np.full((9, 9), 3.0, dtype=np.float32)                            # no signal: centre, N = 100
+    th = oracle.initialize(XYNBSXSY, None, flat, oracle.FP32)
+    assert list(th) == [5.0, 5.0, 100.0, 3.0, pytest.approx(1.3), pytest.approx(1.3)]
+
+
+def test_astig_initial_z_sign(oracle):
+    """z < 0 widens x (sigma_x grows with (z - gamma)^2): the moment estimate must
+    be negative there and positive for the mirrored spot."""
+    from gaussmle_amd import synth
+    psf = synth.ASTIG_CONFIG_PSF
+    for z, sign in [(-250.0, -1), (250.0, 1)]:
+        th = np.array([[8.0], [8.0], [z], [20000.0], [1.0]])
+        mu = synth.expected_counts(psf, th, 15)[:, :, 0]
+        z0 = oracle.initialize(ASTIG, ASTIG_PSF, mu.astype(np.float32), oracle.FP64)[2]
+        assert np.sign(z0) == sign and abs(z0) <= 600
+
+
+# ---------------------------------------------------------------------------
+# Whole fits against an independent maximum-likelihood solver
+# ---------------------------------------------------------------------------
+def _np_model(th, box, model, psf):
+    """Independent numpy model mu[row, col] from scipy's erf."""
+    if model == XYNB:
+        x, y, N, bg = th
+        sx = sy = psf[0]
+    elif model == XYNBS:
+        x, y, N, bg, sx = th
+        sy = sx
+    else:
+        x, y, N, bg, sx, sy = th
+    k = np.arange(1, box + 1)
+    px = np.array([_phi(kk, x, sx) for kk in k])
+    py = np.array([_phi(kk, y, sy) for kk in k])
+    return bg + N * py[:, None] * px[None, :]
+
+
+@pytest.mark.parametrize("model", [XYNB, XYNBS, XYNBSXSY])
+def test_fit_matches_independent_mle(oracle, model):
+    from gaussmle_amd import synth
+    from gaussmle_amd.models import GaussianXYNB, GaussianXYNBS, GaussianXYNBSXSY
+    psfm = {XYNB: GaussianXYNB(1.3), XYNBS: GaussianXYNBS(), XYNBSXSY: GaussianXYNBSXSY()}[model]
+    w = synth.generic(psfm, 6, 11, seed=20 + model, iterations=60, n_range=(3000, 5000), bg_range=(2, 5),
+                      sigma_range=(1.2, 1.5))
+    r = oracle.fit_workload(w, oracle.FP64)
+    psf = [float(np.float32(1.3))] if model == XYNB else None   # the kernel's PSF argument is Float32
+    for k in range(w.n):
+        roi = w.data[:, :, k].astype(np.float64)
+        nll = lambda t: -np.sum(stats.poisson.logpmf(roi, _np_model(t, 11, model, psf)))
+        res = optimize.minimize(nll, r["params"][k], method="Nelder-Mead",
+                                options={"xatol": 1e-9, "fatol": 1e-12, "maxiter": 20000, "maxfev": 40000})
+        assert nll(r["params"][k]) <= res.fun + 1e-6           # the oracle sits at the optimum
+        assert np.abs(r["params"][k][:2] - res.x[:2]).max() < 2e-4
+        # CRLB from an independent Fisher matrix with numeric derivatives
+        t0 = r["params"][k]
+        P = len(t0)
+        mu = _np_model(t0, 11, model, psf)
+        J = []
+        for p in range(P):
+            h = 1e-5 * max(1.0, abs(t0[p]))
+            a, b = t0.copy(), t0.copy()
+            a[p] += h
+            b[p] -= h
+            J.append(((_np_model(a, 11, model, psf) - _np_model(b, 11, model, psf)) / (2 * h)).ravel())
+        J = np.array(J)
+        F = (J / mu.ravel()) @ J.T
+        # the reference's ridge, 1e-10 max|F| on the diagonal (src/unified_kernel.jl:403-406),
+        # moves CRLB_N by ~1e-4 relative because F_NN is ~1e7 times smaller than F_xx
+        F_reg = F + 1e-10 * np.abs(F).max() * np.eye(P)
+        np.testing.assert_allclose(r["crlb"][k], np.sqrt(np.diag(np.linalg.inv(F_reg))), rtol=2e-5)
+        assert abs(r["crlb"][k][2] / np.sqrt(np.linalg.inv(F)[2, 2]) - 1) > 1e-5
+        sat = np.where(roi > 0, stats.poisson.logpmf(roi, np.maximum(roi, 1e-300)), 0.0)
+        assert r["llr"][k] == pytest.approx(np.sum(stats.poisson.logpmf(roi, mu) - sat), rel=1e-8)
+
+
+def test_transpose_symmetry(oracle):
+    from gaussmle_amd import synth
+    from gaussmle_amd.models import GaussianXYNB
+    w = synth.generic(GaussianXYNB(1.3), 40, 11, seed=7)
+    wt = w.subset(40)
+    wt.data = np.asfortranarray(np.transpose(w.data, (1, 0, 2)))
+    a = oracle.fit_workload(w, oracle.FP64)
+    b = oracle.fit_workload(wt, oracle.FP64)
+    np.testing.assert_allclose(a["params"][:, [1, 0, 2, 3]], b["params"], rtol=1e-9, atol=1e-9)
+    np.testing.assert_allclose(a["crlb"][:, [1, 0, 2, 3]], b["crlb"], rtol=1e-8)
+
+
+# ---------------------------------------------------------------------------
+# Statistical efficiency with the reference's tolerances
+# ---------------------------------------------------------------------------
+def _validation_stack(model, n, box=15, seed=42):
+    """test/validation_utils.jl:68-160: N = 1000 +-20%, bg = 5 +-20%, sigma = 1.3,
+    position box/2 + 0.5 randn."""
+    from gaussmle_amd import synth
+    from gaussmle_amd.models import GaussianXYNB, GaussianXYNBS, GaussianXYNBSXSY, default_constraints
+    rng = np.random.default_rng(seed)
+    x = box / 2 + 0.5 * rng.standard_normal(n)
+    y = box / 2 + 0.5 * rng.standard_normal(n)
+    N = 1000 * (0.8 + 0.4 * rng.random(n))
+    bg = 5 * (0.8 + 0.4 * rng.random(n))
+    if model == XYNB:
+        psf, th = GaussianXYNB(1.3), np.stack([x, y, N, bg])
+    elif model == XYNBS:
+        psf, th = GaussianXYNBS(), np.stack([x, y, N, bg, 1.3 * (0.8 + 0.4 * rng.random(n))])
+    else:
+        psf, th = GaussianXYNBSXSY(), np.stack([x, y, N, bg, 1.3 * (0.8 + 0.4 * rng.random(n)),
+                                                 1.3 * (0.8 + 0.4 * rng.random(n))])
+    data = np.asfortranarray(rng.poisson(synth.expected_counts(psf, th, box)).astype(np.float32))
+    return synth.Workload("validation", psf, box, data, th, default_constraints(psf, 11), 20)
+
+
+@pytest.mark.parametrize("model", [XYNB, XYNBS, XYNBSXSY])
+def test_statistical_efficiency(oracle, model):
+    w = _validation_stack(model, 1000)
+    r = oracle.fit_workload(w, oracle.FP32)
+    err = r["params"] - w.truth.T
+    bias_tol = [0.15, 0.15, 100.0, 2.0, 0.05, 0.05]
+    for k in range(err.shape[1]):
+        assert abs(err[:, k].mean()) < bias_tol[k], k
+        ratio = err[:, k].std() / r["crlb"][:, k].mean()
+        assert abs(ratio - 1) < 0.1, (k, ratio)
+
+
+def test_scmos_variance_orientation(oracle):
+    """test/model_validation_tests.jl:283-393: a variance gradient 10x steeper in
+    y than in x; ROIs at (x=400, y=50) must come out more precise than those at
+    (x=50, y=400), by sqrt(var_B / var_A) up to Poisson dilution."""
+    from gaussmle_amd import synth
+    from gaussmle_amd.models import GaussianXYNB, default_constraints
+    cam, box, n = 512, 11, 100
+    ii, jj = np.mgrid[1:cam + 1, 1:cam + 1]
+    var = (1.0 + 0.5 * (ii - 1) + 0.05 * (jj - 1)).astype(np.float32)   # [row = y, col = x]
+    rng = np.random.default_rng(123)
+    th = np.stack([box / 2 + 0.3 * rng.standard_normal(n), box / 2 + 0.3 * rng.standard_normal(n),
+                   np.full(n, 2000.0), np.full(n, 5.0)])
+    xc = np.array([400] * 50 + [50] * 50, dtype=np.int32)
+    yc = np.array([50] * 50 + [400] * 50, dtype=np.int32)
+    mu = synth.expected_counts(GaussianXYNB(1.3), th, box)
+    rows = (yc - 1)[None, None, :] + np.arange(box)[:, None, None]
+    cols = (xc - 1)[None, None, :] + np.arange(box)[None, :, None]
+    data = np.asfortranarray((rng.poisson(mu) + rng.standard_normal(mu.shape) * np.sqrt(var[rows, cols]))
+                             .astype(np.float32))
+    c = default_constraints(GaussianXYNB(1.3), 11)
+    r = oracle.fit(XYNB, [1.3], box, data.reshape(-1, order="F"), c.lower, c.upper, c.max_step, 20,
+                   oracle.FP32, var, xc, yc)
+    ratio = r["crlb"][50:, 0].mean() / r["crlb"][:50, 0].mean()
+    expected = math.sqrt((1 + 0.5 * 406 + 0.05 * 56) / (1 + 0.5 * 56 + 0.05 * 406))
+    assert 1.5 < ratio < 3.0
+    assert abs(ratio - expected) / expected < 0.4
