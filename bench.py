@@ -90,6 +90,20 @@ def cpu_baseline(w, sample: int, threads: int):
     return sub.n / dt, dt
 
 
+def copy_probe(nbytes: int) -> int:
+    """A device-to-device copy of known size, to scale rocprofv3's FETCH_SIZE /
+    WRITE_SIZE counters into bytes."""
+    import torch
+    n = nbytes // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 4, "kernels": 3}), flush=True)
+    return 0
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,7 +115,11 @@ def main(argv=None) -> int:
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--cpu-sample", type=int, default=100_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device copy of BYTES bytes (HBM counter calibration)")
     args = ap.parse_args(argv)
+    if args.copy_probe:
+        return copy_probe(args.copy_probe)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the per-ROI fit shards trivially across GPUs (independent ROIs), "

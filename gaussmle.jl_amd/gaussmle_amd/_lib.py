@@ -7,6 +7,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import sys
 from pathlib import Path
 
 import numpy as np
@@ -75,6 +76,14 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
     if _lib is not None and path is None:
         return _lib
     p = Path(path) if path is not None else LIB_PATH
+    # torch (when installed) bundles its own HIP / HSA runtime under the same
+    # sonames as the system ROCm, and one process can only drive the GPU
+    # through one of them: load torch's first so this library binds to it.
+    if "torch" not in sys.modules and os.environ.get("GMLE_NO_TORCH_PRELOAD") != "1":
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
     if not p.exists():
         raise RuntimeError(
             f"HIP library not found at {p}; build it with `python gaussmle.jl_amd/build.py` "

@@ -12,6 +12,14 @@ for p in (str(ROOT), str(PKG)):
         sys.path.insert(0, p)
 
 
+# torch ships its own HIP / HSA runtime with the same sonames as the system
+# ROCm; loading torch first lets the HIP library bind to that one runtime.
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
+
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X GPU and the built HIP library")
 

@@ -2,14 +2,20 @@
 
 Tolerances (stated once, used by every GPU test):
   * x, y            : |gpu - oracle_fp32| <= 1e-3 px
-  * other parameters: |gpu / oracle_fp64 - 1| <= 1e-4   (z: |gpu - oracle_fp64| <= 1e-3 * max(1, |z|))
+  * other parameters: |gpu / oracle_fp64 - 1| <= 1e-4   (z: |gpu - oracle_fp64| <= 1e-4 * max(1, |z|))
   * CRLB            : |gpu / oracle_fp64 - 1| <= 1e-4
   * LLR             : |gpu - oracle| <= 1e-4 * max(1, |oracle|)   (oracle_fp64 for mixed precision,
                       oracle_fp32 for the all-Float32 mode, whose Float32 LLR formula cancels to ~1e-5)
-Fixed-iteration Newton fits of noisy, low-count ROIs can sit on a clamp or a
-branch edge where a rounding difference sends the two implementations down
-different paths; such ROIs are counted and reported (max, p99.9, count) and
-may not exceed OUTLIER_FRACTION of the batch.
+
+Well-posed ROIs.  A fixed-iteration Newton fit of a noisy or badly initialised
+ROI may not converge; on such an ROI the reference's own Float32 path (the
+Float32 oracle) already misses the Float64 oracle by more than the tolerances
+above, because Float32 rounding sends the two down slightly different
+trajectories.  An ROI is well-posed when the Float32 oracle meets every
+tolerance against the Float64 oracle.  On well-posed ROIs the GPU must meet the
+tolerances (at most OUTLIER_FRACTION of them may miss); on the others the GPU
+result must be finite wherever the Float32 oracle's is, and the deviations are
+reported.
 """
 from __future__ import annotations
 
@@ -26,46 +32,67 @@ def _rel(a, b):
         return np.abs(a / b - 1.0)
 
 
-def compare(model_id: int, gpu: dict, ref32: dict, ref64: dict, precision: str) -> dict:
-    """gpu/ref*: dict(params (n,P), crlb (n,P), llr (n,)).  Returns a report."""
-    gp, gc, gl = (np.asarray(gpu[k], np.float64) for k in ("params", "crlb", "llr"))
+def _errors(model_id: int, res: dict, ref32: dict, ref64: dict, precision: str) -> dict:
+    gp, gc, gl = (np.asarray(res[k], np.float64) for k in ("params", "crlb", "llr"))
     n, P = gp.shape
-    rep = {}
-    bad = np.zeros(n, bool)
-    pos = np.abs(gp[:, :2] - ref32["params"][:, :2]).max(axis=1)
-    rep["pos"] = pos
-    bad |= ~(pos <= POS_TOL)
+    e = {"pos": np.abs(gp[:, :2] - ref32["params"][:, :2]).max(axis=1)}
     for k in range(2, P):
         if model_id == 3 and k == 2:
-            e = np.abs(gp[:, k] - ref64["params"][:, k]) / np.maximum(1.0, np.abs(ref64["params"][:, k]))
+            e[f"p{k}"] = np.abs(gp[:, k] - ref64["params"][:, k]) / np.maximum(1.0, np.abs(ref64["params"][:, k]))
         else:
-            e = _rel(gp[:, k], ref64["params"][:, k])
-        rep[f"p{k}"] = e
-        bad |= ~(e <= REL_TOL)
+            e[f"p{k}"] = _rel(gp[:, k], ref64["params"][:, k])
     ce = _rel(gc, ref64["crlb"]).max(axis=1)
     ce[np.all(np.isinf(gc) & np.isinf(ref64["crlb"]), axis=1)] = 0.0
-    rep["crlb"] = ce
-    bad |= ~(ce <= REL_TOL)
+    e["crlb"] = ce
     lref = ref64["llr"] if precision == "mixed" else ref32["llr"]
-    le = np.abs(gl - lref) / np.maximum(1.0, np.abs(lref))
-    rep["llr"] = le
-    bad |= ~(le <= LLR_TOL)
-    rep["n"] = n
-    rep["bad"] = bad
-    rep["n_bad"] = int(bad.sum())
+    e["llr"] = np.abs(gl - lref) / np.maximum(1.0, np.abs(lref))
+    return e
+
+
+def _bad(e: dict) -> np.ndarray:
+    bad = ~(e["pos"] <= POS_TOL)
+    for k, v in e.items():
+        if k == "pos":
+            continue
+        bad |= ~(v <= (LLR_TOL if k == "llr" else REL_TOL))
+    return bad
+
+
+def compare(model_id: int, gpu: dict, ref32: dict, ref64: dict, precision: str) -> dict:
+    """gpu/ref*: dict(params (n,P), crlb (n,P), llr (n,)).  Returns a report."""
+    e = _errors(model_id, gpu, ref32, ref64, precision)
+    # the reference's own Float32 path against the Float64 oracle (its LLR is
+    # compared like the GPU's in the same precision mode)
+    eref = _errors(model_id, ref32, ref32, ref64, "mixed" if precision == "mixed" else "fp32")
+    if precision == "mixed":
+        eref["llr"] = np.zeros_like(eref["llr"])   # Float32 LLR is not the mixed-mode quantity
+    well = ~_bad(eref)
+    bad_gpu = _bad(e)
+    fin32 = np.all(np.isfinite(ref32["params"]), axis=1)
+    fin_gpu = np.all(np.isfinite(np.asarray(gpu["params"])), axis=1)
+    rep = dict(e)
+    rep["n"] = len(well)
+    rep["n_ill"] = int((~well).sum())
+    rep["bad"] = bad_gpu & well
+    rep["n_bad"] = int(rep["bad"].sum())
+    rep["finite_mismatch"] = int((fin32 & ~fin_gpu).sum())
+    rep["ill_pos_max"] = float(e["pos"][~well].max()) if (~well).any() else 0.0
     return rep
 
 
 def summary(rep: dict) -> str:
-    parts = [f"n={rep['n']} outliers={rep['n_bad']}"]
+    parts = [f"n={rep['n']} ill_posed={rep['n_ill']} outliers={rep['n_bad']} finite_mismatch={rep['finite_mismatch']}"]
+    well = ~(rep["bad"] | False)
     for k, v in rep.items():
         if isinstance(v, np.ndarray) and v.dtype != bool:
             fin = v[np.isfinite(v)]
             if fin.size:
                 parts.append(f"{k}:max={fin.max():.2e},p99.9={np.quantile(fin, 0.999):.2e}")
+    parts.append(f"ill_pos_max={rep['ill_pos_max']:.2e}")
     return " ".join(parts)
 
 
 def assert_parity(rep: dict, label: str = ""):
-    allowed = int(OUTLIER_FRACTION * rep["n"])
+    allowed = int(OUTLIER_FRACTION * (rep["n"] - rep["n_ill"]))
     assert rep["n_bad"] <= allowed, f"{label}: {summary(rep)}"
+    assert rep["finite_mismatch"] == 0, f"{label}: {summary(rep)}"

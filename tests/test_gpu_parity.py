@@ -129,8 +129,11 @@ def test_mixed_precision_tracks_fp64_oracle():
     rmx = reference("astig_ideal_15", refcpu.MIXED)
     r32 = reference("astig_ideal_15", refcpu.FP32)
     gm = gpu_fit_host(w, "mixed", 2)
-    ok = np.abs(gm["params"][:, :2] - rmx["params"][:, :2]).max(axis=1) < 1e-4
-    assert ok.mean() > 0.99
+    # ROIs whose Float32 iterations landed on the same estimate (to Float32
+    # rounding), so that the CRLB difference is the final pass's alone
+    dth = np.abs(gm["params"] / rmx["params"] - 1).max(axis=1)
+    ok = dth < 1e-6
+    assert ok.mean() > 0.9
     e_mixed = np.abs(gm["crlb"][ok] / rmx["crlb"][ok] - 1).max(axis=1)
     e_fp32 = np.abs(r32["crlb"][ok] / rmx["crlb"][ok] - 1).max(axis=1)
     print(f"[mixed] crlb vs mixed oracle p99.9={np.quantile(e_mixed, 0.999):.2e}; "
@@ -157,7 +160,8 @@ def test_zero_iterations_reproduce_initialisation():
         r32 = refcpu.fit_workload(w, refcpu.FP32)
         for lanes in lanes_for(w.box):
             gp = gpu_fit_host(w, "fp32", lanes)["params"]
-            np.testing.assert_allclose(gp, r32["params"], rtol=2e-6, atol=2e-5, err_msg=f"{name} L={lanes}")
+            # Float32 sums in a different order; z amplifies the moment difference
+            np.testing.assert_allclose(gp, r32["params"], rtol=1e-4, atol=1e-4, err_msg=f"{name} L={lanes}")
 
 
 def test_statistical_efficiency_at_scale():

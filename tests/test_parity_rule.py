@@ -1,0 +1,50 @@
+"""The GPU parity checker (tests/parity.py) itself, exercised on the CPU with
+mistakes planted in copies of the oracle's output."""
+import numpy as np
+import pytest
+
+from parity import assert_parity, compare
+
+
+@pytest.fixture(scope="module")
+def refs(oracle):
+    from gaussmle_amd import synth
+    w = synth.config0(n=600, seed=77)
+    return oracle.fit_workload(w, oracle.FP32), oracle.fit_workload(w, oracle.FP64)
+
+
+def _copy(r):
+    return {k: np.array(v, copy=True) for k, v in r.items()}
+
+
+def test_reference_passes_itself(refs):
+    r32, r64 = refs
+    rep = compare(0, _copy(r32), r32, r64, "fp32")
+    assert rep["n_bad"] == 0 and rep["finite_mismatch"] == 0
+    assert_parity(rep)
+
+
+@pytest.mark.parametrize("field,k,delta", [("params", 0, 2e-3), ("params", 1, -2e-3), ("params", 2, 3e-4),
+                                           ("crlb", 3, 3e-4), ("llr", None, 3e-4)])
+def test_planted_errors_are_flagged(refs, field, k, delta):
+    r32, r64 = refs
+    g = _copy(r32)
+    rep0 = compare(0, g, r32, r64, "fp32")
+    roi = int(np.where(~rep0["bad"])[0][0])
+    if field == "llr":
+        g["llr"][roi] += delta * max(1.0, abs(g["llr"][roi]))
+    elif k < 2 and field == "params":
+        g[field][roi, k] += delta
+    else:
+        g[field][roi, k] *= 1 + delta
+    rep = compare(0, g, r32, r64, "fp32")
+    assert rep["bad"][roi] and rep["n_bad"] == 1
+
+
+def test_nan_where_reference_is_finite_fails(refs):
+    r32, r64 = refs
+    g = _copy(r32)
+    g["params"][5, :] = np.nan
+    rep = compare(0, g, r32, r64, "fp32")
+    with pytest.raises(AssertionError):
+        assert_parity(rep)
