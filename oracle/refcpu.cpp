@@ -186,4 +186,32 @@ int refcpu_fit(int prec, int model, const float* psf, int box, int64_t n_rois, c
   return 0;
 }
 
+// The final pass at given estimates theta (n_rois x P, Float64): LLR, Fisher,
+// inversion and CRLB in Float32 (prec 0) or Float64 (prec 1 / 2).
+int refcpu_final_pass(int prec, int model, const float* psf, int box, int64_t n_rois, const float* data,
+                      int camera, const float* var_map, int ny, int nx, const int32_t* x_corners,
+                      const int32_t* y_corners, const double* theta, double* crlb, double* llr, int nthreads) {
+  if (box < 3 || box > 32 || n_rois < 0 || model < 0 || model > 3) return -1;
+  if (camera && (!var_map || !x_corners || !y_corners)) return -1;
+  (void)nx;
+  const int P = n_params(model);
+  const Psf<float> p = make_psf(model, psf);
+  const int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nt)
+  for (int64_t r = 0; r < n_rois; ++r) {
+    const float* roi = data + (size_t)r * box * box;
+    const int cx = x_corners ? x_corners[r] : 1;
+    const int cy = y_corners ? y_corners[r] : 1;
+    if (prec == 0) {
+      float cr[6], l;
+      final_pass<float>(roi, box, p, camera != 0, var_map, ny, cx, cy, theta + r * P, cr, &l);
+      for (int k = 0; k < P; ++k) crlb[r * P + k] = cr[k];
+      llr[r] = l;
+    } else {
+      final_pass<double>(roi, box, p, camera != 0, var_map, ny, cx, cy, theta + r * P, crlb + r * P, llr + r);
+    }
+  }
+  return 0;
+}
+
 }  // extern "C"

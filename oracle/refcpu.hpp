@@ -539,4 +539,40 @@ inline int fit_roi(const float* roi_f32, int n, const Psf<float>& psf_f32, bool 
   return path;
 }
 
+// The final pass alone (src/unified_kernel.jl:353-431) at a given estimate:
+// LLR, Fisher matrix, inversion and CRLB in arithmetic type TF.
+template <class TF>
+inline int final_pass(const float* roi_f32, int n, const Psf<float>& psf_f32, bool scmos, const float* var,
+                      int ny, int cx, int cy, const double* theta, TF* crlb_out, TF* llr_out) {
+  const int P = n_params(psf_f32.model);
+  const Psf<TF> psf2 = psf_cast<TF>(psf_f32);
+  auto var_at = [&](int i, int j) -> float {
+    const int cam_i = cy + i - 1, cam_j = cx + j - 1;
+    return var[(size_t)(cam_i - 1) + (size_t)ny * (size_t)(cam_j - 1)];
+  };
+  TF th2[6], du2[6], dd2[6];
+  for (int k = 0; k < P; ++k) th2[k] = (TF)theta[k];
+  TF llr = TF(0);
+  TF F[36];
+  for (int q = 0; q < 36; ++q) F[q] = TF(0);
+  for (int j = 1; j <= n; ++j)
+    for (int i = 1; i <= n; ++i) {
+      const TF model = pixel_derivatives(i, j, th2, psf2, du2, dd2);
+      const TF data = (TF)roi_f32[(i - 1) + n * (j - 1)];
+      if (!scmos) llr += llr_poisson(data, model);
+      else llr += llr_scmos(data, model, (TF)var_at(i, j));
+      if (model > TF(0)) {
+        const TF v = scmos ? model + (TF)var_at(i, j) : model;
+        for (int k = 0; k < P; ++k)
+          for (int l = k; l < P; ++l) {
+            const TF f = du2[k] * du2[l] / v;
+            F[k + P * l] += f;
+            if (k != l) F[l + P * k] += f;
+          }
+      }
+    }
+  *llr_out = llr;
+  return invert_fisher(F, P, crlb_out);
+}
+
 }  // namespace refcpu

@@ -60,6 +60,9 @@ def load() -> C.CDLL:
         lib.refcpu_fit.restype = C.c_int
         lib.refcpu_fit.argtypes = [C.c_int, C.c_int, _F, C.c_int, C.c_int64, _F, C.c_int, _F, C.c_int, C.c_int,
                                    _I, _I, _F, _F, _F, C.c_int, _D, _D, _D, _I, C.c_int]
+        lib.refcpu_final_pass.restype = C.c_int
+        lib.refcpu_final_pass.argtypes = [C.c_int, C.c_int, _F, C.c_int, C.c_int64, _F, C.c_int, _F, C.c_int,
+                                          C.c_int, _I, _I, _D, _D, _D, C.c_int]
         _lib = lib
     return _lib
 
@@ -182,3 +185,31 @@ def fit_workload(w, prec: int = FP32, nthreads: int = 0, n: int | None = None):
     c = ww.constraints
     return fit(ww.psf_pixels.model_id, ww.psf_pixels.psf_array(), ww.box, ww.flat_data(), c.lower, c.upper,
                c.max_step, ww.iterations, prec, ww.var_map, ww.x_corners, ww.y_corners, nthreads)
+
+
+def final_pass(w, theta: np.ndarray, prec: int = FP64, nthreads: int = 0):
+    """LLR / Fisher / CRLB of a synth.Workload at given estimates theta (n, P)."""
+    lib = load()
+    P = len(w.psf_pixels)
+    th = np.ascontiguousarray(theta, dtype=np.float64)
+    n = th.shape[0]
+    crlb = np.zeros(n * P)
+    llr = np.zeros(n)
+    d = np.ascontiguousarray(w.flat_data()[: n * w.box * w.box], dtype=np.float32)
+    camera = 0 if w.var_map is None else 1
+    ny = nx = 0
+    vflat = None
+    if camera:
+        v = np.asfortranarray(w.var_map, dtype=np.float32)
+        ny, nx = v.shape
+        vflat = v.reshape(-1, order="F")
+    xc = None if w.x_corners is None else np.ascontiguousarray(w.x_corners[:n], dtype=np.int32)
+    yc = None if w.y_corners is None else np.ascontiguousarray(w.y_corners[:n], dtype=np.int32)
+    psf = _psf(w.psf_pixels.model_id, w.psf_pixels.psf_array())
+    rc = lib.refcpu_final_pass(prec, w.psf_pixels.model_id, psf.ctypes.data_as(_F), w.box, n, d.ctypes.data_as(_F),
+                               camera, _f(vflat), ny, nx, None if xc is None else xc.ctypes.data_as(_I),
+                               None if yc is None else yc.ctypes.data_as(_I), th.ctypes.data_as(_D),
+                               crlb.ctypes.data_as(_D), llr.ctypes.data_as(_D), nthreads)
+    if rc != 0:
+        raise ValueError("refcpu_final_pass: bad arguments")
+    return {"crlb": crlb.reshape(n, P), "llr": llr}

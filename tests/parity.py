@@ -2,10 +2,14 @@
 
 Tolerances (stated once, used by every GPU test):
   * x, y            : |gpu - oracle_fp32| <= 1e-3 px
-  * other parameters: |gpu / oracle_fp64 - 1| <= 1e-4   (z: |gpu - oracle_fp64| <= 1e-4 * max(1, |z|))
+  * other parameters: |gpu - oracle_fp64| <= 1e-4 * max(|oracle_fp64|, floor), floor = 1 for z and
+                      bg (photons / pixel) and 0 for N and widths, i.e. relative 1e-4 except that
+                      a background or z within 1 of zero is held to 1e-4 absolute
   * CRLB            : |gpu / oracle_fp64 - 1| <= 1e-4
-  * LLR             : |gpu - oracle| <= 1e-4 * max(1, |oracle|)   (oracle_fp64 for mixed precision,
-                      oracle_fp32 for the all-Float32 mode, whose Float32 LLR formula cancels to ~1e-5)
+  * LLR             : mixed precision: |gpu - oracle_fp64| <= 1e-4 * max(1, |oracle_fp64|);
+                      Float32 mode: |gpu - oracle_fp32| <= max(1e-4 |oracle_fp32|, 1e-3), since the
+                      reference's Float32 formula d log(mu) - mu - (d log(d) - d) cancels terms of
+                      size d log(d) and carries ~1e-4 absolute rounding noise
 
 Well-posed ROIs.  A fixed-iteration Newton fit of a noisy or badly initialised
 ROI may not converge; on such an ROI the reference's own Float32 path (the
@@ -36,16 +40,19 @@ def _errors(model_id: int, res: dict, ref32: dict, ref64: dict, precision: str) 
     gp, gc, gl = (np.asarray(res[k], np.float64) for k in ("params", "crlb", "llr"))
     n, P = gp.shape
     e = {"pos": np.abs(gp[:, :2] - ref32["params"][:, :2]).max(axis=1)}
+    ibg = 4 if model_id == 3 else 3
     for k in range(2, P):
-        if model_id == 3 and k == 2:
-            e[f"p{k}"] = np.abs(gp[:, k] - ref64["params"][:, k]) / np.maximum(1.0, np.abs(ref64["params"][:, k]))
-        else:
-            e[f"p{k}"] = _rel(gp[:, k], ref64["params"][:, k])
+        floor = 1.0 if (k == ibg or (model_id == 3 and k == 2)) else 0.0
+        with np.errstate(divide="ignore", invalid="ignore"):
+            e[f"p{k}"] = np.abs(gp[:, k] - ref64["params"][:, k]) / np.maximum(floor, np.abs(ref64["params"][:, k]))
     ce = _rel(gc, ref64["crlb"]).max(axis=1)
     ce[np.all(np.isinf(gc) & np.isinf(ref64["crlb"]), axis=1)] = 0.0
     e["crlb"] = ce
-    lref = ref64["llr"] if precision == "mixed" else ref32["llr"]
-    e["llr"] = np.abs(gl - lref) / np.maximum(1.0, np.abs(lref))
+    if precision == "mixed":
+        e["llr"] = np.abs(gl - ref64["llr"]) / np.maximum(1.0, np.abs(ref64["llr"]))
+    else:
+        # expressed in units of the tolerance scale, compared against LLR_TOL
+        e["llr"] = np.abs(gl - ref32["llr"]) / np.maximum(np.abs(ref32["llr"]), 10.0)
     return e
 
 

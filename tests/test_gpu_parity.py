@@ -121,24 +121,28 @@ def test_parity_device_path(name):
         assert_parity(rep, f"device {name} L={lanes}")
 
 
-def test_mixed_precision_tracks_fp64_oracle():
-    """The Float64 epilogue matches the mixed-precision oracle (same Float32
-    iterations, Float64 final pass) far more tightly than the Float32 one."""
+@pytest.mark.parametrize("name", ["astig_ideal_15", "xynbs_scmos_13", "xynb_ideal_11"])
+def test_final_pass_against_oracle_at_same_estimate(name):
+    """The epilogue alone: the oracle's final pass (LLR, Fisher, inversion,
+    CRLB) evaluated at the GPU's own estimates.  Mixed precision must match the
+    Float64 final pass to near Float64 rounding; the Float32 mode matches the
+    Float32 final pass and is reported against the Float64 one."""
     from oracle import refcpu
-    w = workload("astig_ideal_15")
-    rmx = reference("astig_ideal_15", refcpu.MIXED)
-    r32 = reference("astig_ideal_15", refcpu.FP32)
-    gm = gpu_fit_host(w, "mixed", 2)
-    # ROIs whose Float32 iterations landed on the same estimate (to Float32
-    # rounding), so that the CRLB difference is the final pass's alone
-    dth = np.abs(gm["params"] / rmx["params"] - 1).max(axis=1)
-    ok = dth < 1e-6
-    assert ok.mean() > 0.9
-    e_mixed = np.abs(gm["crlb"][ok] / rmx["crlb"][ok] - 1).max(axis=1)
-    e_fp32 = np.abs(r32["crlb"][ok] / rmx["crlb"][ok] - 1).max(axis=1)
-    print(f"[mixed] crlb vs mixed oracle p99.9={np.quantile(e_mixed, 0.999):.2e}; "
-          f"fp32 oracle vs mixed oracle p99.9={np.quantile(e_fp32, 0.999):.2e}", flush=True)
-    assert np.quantile(e_mixed, 0.999) < 1e-4
+    w = workload(name)
+    gm = gpu_fit_host(w, "mixed", 0)
+    f64 = refcpu.final_pass(w, gm["params"], refcpu.FP64)
+    e_crlb = np.abs(gm["crlb"] / f64["crlb"] - 1).max(axis=1)
+    e_llr = np.abs(gm["llr"] - f64["llr"]) / np.maximum(1.0, np.abs(f64["llr"]))
+    g32 = gpu_fit_host(w, "fp32", 0)
+    f32 = refcpu.final_pass(w, g32["params"], refcpu.FP32)
+    f64b = refcpu.final_pass(w, g32["params"], refcpu.FP64)
+    e32 = np.abs(g32["crlb"] / f32["crlb"] - 1).max(axis=1)
+    e32_64 = np.abs(g32["crlb"] / f64b["crlb"] - 1).max(axis=1)
+    print(f"[final] {name} mixed vs fp64 final pass: crlb max={e_crlb.max():.2e} p99.9={np.quantile(e_crlb, .999):.2e} "
+          f"llr max={e_llr.max():.2e}; fp32 vs fp32 final pass crlb p99.9={np.quantile(e32, .999):.2e}; "
+          f"fp32 vs fp64 final pass crlb max={e32_64.max():.2e} p99.9={np.quantile(e32_64, .999):.2e}", flush=True)
+    assert e_crlb.max() < 1e-9 and e_llr.max() < 1e-9
+    assert np.quantile(e32, 0.999) < 1e-4
 
 
 @pytest.mark.parametrize("n", [1, 63, 65, 129])
