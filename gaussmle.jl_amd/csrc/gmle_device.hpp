@@ -52,10 +52,8 @@ template <int L> __device__ __forceinline__ float group_max(float v) {
 }
 
 // ---------------------------------------------------------------------------
-// Transcendentals.  exp(-t^2/(2 s^2)) is evaluated as exp2(-(t*k)^2) with
-// k = sqrt(log2(e)/2)/s, a single v_exp_f32.  1/x is v_rcp_f32 (1 ulp).
+// Transcendentals: exp2 is a single v_exp_f32, 1/x a single v_rcp_f32 (1 ulp).
 // ---------------------------------------------------------------------------
-constexpr float kSqrtHalfLog2e = 0.84932180028801904272f;   // sqrt(log2(e) / 2)
 constexpr float kInvSqrt2Pi = 0.39894228040143267794f;      // 1 / sqrt(2 pi)
 constexpr double kInvSqrt2PiD = 0.39894228040143267794;
 constexpr double kHalfD = 0.5;

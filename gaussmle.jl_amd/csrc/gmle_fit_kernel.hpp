@@ -37,6 +37,7 @@
 
 #include "gmle_args.hpp"
 #include "gmle_device.hpp"
+#include "gmle_erf.hpp"
 
 namespace gmle {
 
@@ -64,22 +65,20 @@ template <> struct MT<ASTIG> {
 // ---------------------------------------------------------------------------
 template <class T> struct DM;
 template <> struct DM<float> {
-  static __device__ __forceinline__ float erf_(float x) { return erff(x); }
-  // exp(-t^2 / (2 s^2)) given k = sqrt(log2(e)/2) / s
-  static __device__ __forceinline__ float gauss(float t, float k) {
-    const float a = t * k;
-    return fast_exp2(-(a * a));
+  // erf(u) and exp(-u^2) for u = t * s; the erf reuses the Gaussian (gmle_erf.hpp)
+  static __device__ __forceinline__ void erf_gauss(float t, float s, float& ev, float& g) {
+    const float u = t * s;
+    g = fast_exp2(-(u * u) * kErfLog2e);
+    ev = erf_with_gauss(u, g);
   }
-  static __device__ __forceinline__ float kexp(float inv_s) { return kSqrtHalfLog2e * inv_s; }
   static __device__ __forceinline__ float inv_sqrt_2pi() { return kInvSqrt2Pi; }
 };
 template <> struct DM<double> {
-  static __device__ __forceinline__ double erf_(double x) { return erf(x); }
-  static __device__ __forceinline__ double gauss(double t, double inv_s) {
-    const double a = t * inv_s;
-    return exp(-0.5 * (a * a));
+  static __device__ __forceinline__ void erf_gauss(double t, double s, double& ev, double& g) {
+    const double u = t * s;
+    g = exp(-(u * u));
+    ev = erf(u);
   }
-  static __device__ __forceinline__ double kexp(double inv_s) { return inv_s; }
   static __device__ __forceinline__ double inv_sqrt_2pi() { return kInvSqrt2PiD; }
 };
 
@@ -104,7 +103,6 @@ template <class T, int M, bool SIG>
 __device__ __forceinline__ void axis_tables(T mu, T sigma, T nph, int first, AxisTab<T, M>& tb) {
   const T inv_s = T(1) / sigma;
   const T s_erf = sqrt(T(0.5) / (sigma * sigma));
-  const T kx = DM<T>::kexp(inv_s);
   const T c = -nph * DM<T>::inv_sqrt_2pi() * inv_s;
   const T c2 = c * inv_s * inv_s;
   const T cs = c * inv_s;
@@ -114,8 +112,8 @@ __device__ __forceinline__ void axis_tables(T mu, T sigma, T nph, int first, Axi
 #pragma unroll
   for (int e = 0; e <= M; ++e) {
     const T t = (T(first + e) + T(0.5)) - mu;
-    const T ev = DM<T>::erf_(t * s_erf);
-    const T g = DM<T>::gauss(t, kx);
+    T ev, g;
+    DM<T>::erf_gauss(t, s_erf, ev, g);
     const T tg = t * g;
     const T t3g = SIG ? t * t * tg : T(0);
     if (e > 0) {
@@ -139,7 +137,7 @@ __device__ __forceinline__ void axis_tables(T mu, T sigma, T nph, int first, Axi
 // Axis constants of axis_tables() and single-edge evaluation, for computing
 // one pixel's tables from its two edges on the fly (column side).
 template <class T> struct AxisK {
-  T mu, s_erf, kx, c, c2, cs, css, m2s;
+  T mu, s_erf, c, c2, cs, css, m2s;
 };
 template <class T> struct Edge {
   T ev, g, tg, t3g;
@@ -150,7 +148,6 @@ __device__ __forceinline__ AxisK<T> axis_consts(T mu, T sigma, T nph) {
   const T inv_s = T(1) / sigma;
   k.mu = mu;
   k.s_erf = sqrt(T(0.5) / (sigma * sigma));
-  k.kx = DM<T>::kexp(inv_s);
   k.c = -nph * DM<T>::inv_sqrt_2pi() * inv_s;
   k.c2 = k.c * inv_s * inv_s;
   k.cs = k.c * inv_s;
@@ -162,8 +159,7 @@ template <class T, bool SIG>
 __device__ __forceinline__ Edge<T> edge_eval(const AxisK<T>& k, int e) {
   Edge<T> r;
   const T t = (T(e) + T(0.5)) - k.mu;
-  r.ev = DM<T>::erf_(t * k.s_erf);
-  r.g = DM<T>::gauss(t, k.kx);
+  DM<T>::erf_gauss(t, k.s_erf, r.ev, r.g);
   r.tg = t * r.g;
   r.t3g = SIG ? t * t * r.tg : T(0);
   return r;
@@ -230,6 +226,174 @@ __device__ __forceinline__ void model_widths(const T* th, const float* psf, Widt
   }
 }
 
+// Fold one column's row-weighted sums S[0..10] = S1..S11 (see accumulate)
+// into the gradient / diagonal Hessian with the column's tables
+// (unified_kernel.jl:330-333): grad_k += dudt_k cf, H_k += d2_k cf - dudt_k^2 df.
+template <int MODEL>
+__device__ __forceinline__ void fold_column(const Pix<float>& cx, const Widths<float>& w, const float* S, float* G,
+                                            float* H) {
+  using Tr = MT<MODEL>;
+  constexpr bool SIG = Tr::EXTRA != 0;
+  constexpr bool TWO = (Tr::EXTRA == 1 || Tr::EXTRA == 3);
+  const float px = cx.phi, Dx = cx.D, D2x = cx.D2;
+  G[Tr::IX] += Dx * S[0];
+  H[Tr::IX] += D2x * S[0] - Dx * Dx * S[1];
+  G[Tr::IY] += px * S[2];
+  H[Tr::IY] += px * S[3] - px * px * S[4];
+  G[Tr::IN] += px * S[0];
+  H[Tr::IN] -= px * px * S[1];
+  G[Tr::IBG] += S[5];
+  H[Tr::IBG] -= S[6];
+  if constexpr (SIG) {
+    float ax, aax;
+    if constexpr (MODEL == ASTIG) {
+      ax = w.dsx * cx.S;
+      aax = cx.SS * (w.dsx * w.dsx) + cx.S * w.d2sx;
+    } else {
+      ax = cx.S;
+      aax = cx.SS;
+    }
+    if constexpr (TWO) {
+      G[Tr::IE1] += ax * S[0] + px * S[7];
+      H[Tr::IE1] += aax * S[0] + px * S[8] - ax * ax * S[1] - 2.f * ax * px * S[9] - px * px * S[10];
+    } else {
+      G[Tr::IE1] += ax * S[0];
+      H[Tr::IE1] += aax * S[0] - ax * ax * S[1];
+      G[Tr::IE2] += px * S[7];
+      H[Tr::IE2] += px * S[8] - px * px * S[10];
+    }
+  }
+}
+
+// The same sums with rows in the outer loop: each row's tables are built on
+// the fly from its two edges and the CB columns of this lane keep their S
+// vectors live instead.  Cheaper in VGPRs than holding every row's tables
+// when there are many row tables (sigma / z models) and few columns per lane.
+template <int MODEL, int CAM, int N, int L, bool FAST>
+__device__ __forceinline__ void accumulate_rows(const float* dv, const float* vr, int c0, float nph, float bg,
+                                                const AxisK<float>& kr, const AxisK<float>& kc,
+                                                const Widths<float>& w, float* G, float* H) {
+  using Tr = MT<MODEL>;
+  constexpr int P = Tr::P;
+  constexpr int CB = (N + L - 1) / L;
+  constexpr bool SIG = Tr::EXTRA != 0;
+  constexpr bool TWO = (Tr::EXTRA == 1 || Tr::EXTRA == 3);
+  constexpr int NS = SIG ? 11 : 7;
+#pragma unroll
+  for (int k = 0; k < P; ++k) { G[k] = 0.f; H[k] = 0.f; }
+  // this lane's column tables
+  Pix<float> cx[CB];
+  float ncx[CB];
+  {
+    Edge<float> lo = edge_eval<float, SIG>(kc, c0);
+#pragma unroll
+    for (int cc = 0; cc < CB; ++cc) {
+      const Edge<float> hi = edge_eval<float, SIG>(kc, c0 + cc + 1);
+      cx[cc] = pix_from_edges<float, SIG>(kc, lo, hi);
+      lo = hi;
+      ncx[cc] = nph * cx[cc].phi;
+    }
+  }
+  float S[CB][11];
+#pragma unroll
+  for (int cc = 0; cc < CB; ++cc)
+#pragma unroll
+    for (int q = 0; q < 11; ++q) S[cc][q] = 0.f;
+  float sphi = 0.f, sD = 0.f, sD2 = 0.f, sA = 0.f, sAA = 0.f;
+  Edge<float> rlo = edge_eval<float, SIG>(kr, 0);
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    // Keep each row's work together: without this the scheduler hoists later
+    // rows' edge evaluations and every row's tables become live at once.
+    __builtin_amdgcn_sched_barrier(0);
+    const Edge<float> rhi = edge_eval<float, SIG>(kr, i + 1);
+    const Pix<float> ry = pix_from_edges<float, SIG>(kr, rlo, rhi);
+    rlo = rhi;
+    float ay = 0.f, aay = 0.f;
+    if constexpr (SIG) {
+      if constexpr (MODEL == ASTIG) {
+        ay = w.dsy * ry.S;
+        aay = ry.SS * (w.dsy * w.dsy) + ry.S * w.d2sy;
+      } else {
+        ay = ry.S;
+        aay = ry.SS;
+      }
+    }
+    if constexpr (CAM == IDEAL && FAST) {
+      sphi += ry.phi;
+      sD += ry.D;
+      sD2 += ry.D2;
+      if constexpr (SIG) { sA += ay; sAA += aay; }
+    }
+    const float py = ry.phi, py2 = py * py, Dy2 = ry.D * ry.D, a2 = ay * ay, pa = py * ay;
+#pragma unroll
+    for (int cc = 0; cc < CB; ++cc) {
+      const float d = dv[cc * N + i];
+      const float mu = fmaf(ncx[cc], py, bg);
+      float cf, df;
+      if constexpr (CAM == IDEAL) {
+        const float rc = fast_rcp(mu);
+        const float q = d * rc;
+        if constexpr (FAST) {
+          cf = q;   // cf + 1; the constant is removed after the row sums
+          df = q * rc;
+        } else {
+          const bool ok = mu > 0.01f;   // camera_models.jl:23-32
+          cf = ok ? jmin(q - 1.f, 1.0e4f) : 0.f;
+          df = ok ? jmin(q * rc, 1.0e4f) : 0.f;
+        }
+      } else {
+        const float v = mu + vr[cc * N + i];   // camera_models.jl:36-42
+        const float rc = fast_rcp(v);
+        cf = (d - mu) * rc;
+        df = rc;
+      }
+      S[cc][0] = fmaf(py, cf, S[cc][0]);
+      S[cc][1] = fmaf(py2, df, S[cc][1]);
+      S[cc][2] = fmaf(ry.D, cf, S[cc][2]);
+      S[cc][3] = fmaf(ry.D2, cf, S[cc][3]);
+      S[cc][4] = fmaf(Dy2, df, S[cc][4]);
+      S[cc][5] += cf;
+      S[cc][6] += df;
+      if constexpr (SIG) {
+        S[cc][7] = fmaf(ay, cf, S[cc][7]);
+        S[cc][8] = fmaf(aay, cf, S[cc][8]);
+        S[cc][10] = fmaf(a2, df, S[cc][10]);
+        if constexpr (TWO) S[cc][9] = fmaf(pa, df, S[cc][9]);
+      }
+    }
+  }
+#pragma unroll
+  for (int cc = 0; cc < CB; ++cc) {
+    if constexpr (CAM == IDEAL && FAST) {
+      S[cc][0] -= sphi;
+      S[cc][2] -= sD;
+      S[cc][3] -= sD2;
+      S[cc][5] -= (float)N;
+      if constexpr (SIG) { S[cc][7] -= sA; S[cc][8] -= sAA; }
+    }
+    const bool cvalid = (L == 1) || (c0 + cc < N);
+    if (!cvalid) {
+#pragma unroll
+      for (int q = 0; q < 11; ++q) S[cc][q] = 0.f;
+    }
+    fold_column<MODEL>(cx[cc], w, S[cc], G, H);
+  }
+  (void)NS;
+}
+
+// Loop order with the smaller live-register estimate: column-outer holds every
+// row's tables (5 per row, 9 with sigma / z terms, 8 for separate sigmas);
+// row-outer holds this lane's column tables and S vectors (about 11 and 18
+// per column).
+template <int MODEL, int CAM, int N, int L> constexpr bool row_outer() {
+  constexpr int CB = (N + L - 1) / L;
+  constexpr int ex = MT<MODEL>::EXTRA;
+  constexpr int row_tables = ex == 0 ? 5 : (ex == 2 ? 8 : 9);
+  constexpr int per_col = ex == 0 ? 11 : 18;
+  return per_col * CB < row_tables * N;
+}
+
 // ---------------------------------------------------------------------------
 // One Newton iteration's gradient G[P] and diagonal Hessian H[P] over this
 // lane's column block (unified_kernel.jl:319-334):
@@ -279,7 +443,7 @@ __device__ __forceinline__ void accumulate(const float* dv, const float* vr, int
         const float rc = fast_rcp(mu);
         const float q = d * rc;
         if constexpr (FAST) {
-          cf = q;               // cf + 1; the constant is removed after the row sums
+          cf = q;   // cf + 1; the constant is removed after the row sums
           df = q * rc;
         } else {
           const bool ok = mu > 0.01f;   // camera_models.jl:23-32
@@ -317,35 +481,173 @@ __device__ __forceinline__ void accumulate(const float* dv, const float* vr, int
     if (!cvalid) {
       S1 = S2 = S3 = S4 = S5 = S6 = S7 = S8 = S9 = S10 = S11 = 0.f;
     }
-    const float px = cx.phi, Dx = cx.D, D2x = cx.D2;
-    G[Tr::IX] += Dx * S1;
-    H[Tr::IX] += D2x * S1 - Dx * Dx * S2;
-    G[Tr::IY] += px * S3;
-    H[Tr::IY] += px * S4 - px * px * S5;
-    G[Tr::IN] += px * S1;
-    H[Tr::IN] -= px * px * S2;
-    G[Tr::IBG] += S6;
-    H[Tr::IBG] -= S7;
-    if constexpr (SIG) {
-      float ax, aax;
-      if constexpr (MODEL == ASTIG) {
-        ax = w.dsx * cx.S;
-        aax = cx.SS * (w.dsx * w.dsx) + cx.S * w.d2sx;
-      } else {
-        ax = cx.S;
-        aax = cx.SS;
+    const float Sv[11] = {S1, S2, S3, S4, S5, S6, S7, S8, S9, S10, S11};
+    fold_column<MODEL>(cx, w, Sv, G, H);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Final pass (unified_kernel.jl:353-431): at the estimate th, the LLR
+// (camera_models.jl:60-85), the full Fisher matrix over pixels with model > 0
+// (unified_kernel.jl:378-396) and its inversion into the CRLB, in arithmetic
+// type TF.  Pixel data come from Src: VGPR arrays (RegSrc, in the iteration
+// kernel) or global memory (GlobalSrc, in the Float64 final-pass kernel).
+// ---------------------------------------------------------------------------
+template <int N, bool SC> struct RegSrc {
+  static constexpr bool kInRegisters = true;
+  const float* dv;   // this lane's column block, [cc * N + i]
+  const float* vr;
+  __device__ __forceinline__ float d(int cc, int i) const { return dv[cc * N + i]; }
+  __device__ __forceinline__ float var(int cc, int i) const { return SC ? vr[cc * N + i] : 0.f; }
+};
+template <int N, bool SC> struct GlobalSrc {
+  static constexpr bool kInRegisters = false;
+  const float* data = nullptr;   // the ROI, column-major
+  const float* var = nullptr;    // var_map at the ROI's first pixel
+  int ny = 0;
+  __device__ __forceinline__ float d(int cc, int i) const { return data[cc * N + i]; }
+  __device__ __forceinline__ float var_at(int cc, int i) const { return var[(int64_t)cc * ny + i]; }
+};
+template <int N, bool SC> __device__ __forceinline__ float src_var(const RegSrc<N, SC>& s, int cc, int i) {
+  return s.var(cc, i);
+}
+template <int N, bool SC> __device__ __forceinline__ float src_var(const GlobalSrc<N, SC>& s, int cc, int i) {
+  return s.var_at(cc, i);
+}
+
+template <class TF, int P> struct FinalOut {
+  TF crlb[P];
+  TF llr;
+};
+
+template <int MODEL, int CAM, int N, int L, class TF, class Src>
+__device__ __forceinline__ FinalOut<TF, MT<MODEL>::P> final_pass(const float* th, const float* psf, int c0,
+                                                               const Src& src) {
+  using Tr = MT<MODEL>;
+  constexpr int P = Tr::P;
+  constexpr int CB = (N + L - 1) / L;
+  constexpr bool SIG = Tr::EXTRA != 0;
+  TF thf[P];
+#pragma unroll
+  for (int k = 0; k < P; ++k) thf[k] = (TF)th[k];
+  Widths<TF> wf;
+  model_widths<MODEL, TF>(thf, psf, wf);
+  const TF nphf = thf[Tr::IN], bgf = thf[Tr::IBG];
+  AxisTab<TF, N> fy;
+  axis_tables<TF, N, SIG>(thf[Tr::IY], wf.sy, nphf, 0, fy);
+  const AxisK<TF> kf = axis_consts<TF>(thf[Tr::IX], wf.sx, nphf);
+
+  constexpr int NF = P * (P + 1) / 2;
+  TF Fu[NF];
+#pragma unroll
+  for (int q = 0; q < NF; ++q) Fu[q] = TF(0);
+  TF llr = TF(0);
+  Edge<TF> flo = edge_eval<TF, false>(kf, c0);
+  auto column = [&](int cc) __attribute__((always_inline)) {
+    const Edge<TF> fhi = edge_eval<TF, false>(kf, c0 + cc + 1);
+    const Pix<TF> fx = pix_from_edges<TF, SIG>(kf, flo, fhi);
+    flo = fhi;
+    const TF px = fx.phi;
+    const bool cvalid = (L == 1) || (c0 + cc < N);
+    TF ax = TF(0);
+    if constexpr (SIG) ax = (MODEL == ASTIG) ? wf.dsx * fx.S : fx.S;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const TF py = fy.phi[i];
+      const TF d = (TF)src.d(cc, i);
+      const TF mu = bgf + nphf * px * py;
+      TF du[P];
+      du[Tr::IX] = fx.D * py;
+      du[Tr::IY] = fy.D[i] * px;
+      du[Tr::IN] = px * py;
+      du[Tr::IBG] = TF(1);
+      if constexpr (SIG) {
+        const TF ay = (MODEL == ASTIG) ? wf.dsy * fy.S[i] : fy.S[i];
+        if constexpr (Tr::EXTRA == 2) {
+          du[Tr::IE1] = ax * py;
+          du[Tr::IE2] = ay * px;
+        } else {
+          du[Tr::IE1] = ax * py + ay * px;
+        }
       }
-      if constexpr (TWO) {
-        G[Tr::IE1] += ax * S1 + px * S8;
-        H[Tr::IE1] += aax * S1 + px * S9 - ax * ax * S2 - 2.f * ax * px * S10 - px * px * S11;
+      TF v = mu;
+      TF lp;
+      if constexpr (CAM == IDEAL) {
+        // camera_models.jl:60-71; in Float64 as d log(mu/d) + d - mu (one log)
+        if constexpr (sizeof(TF) == 8) {
+          lp = (mu > TF(0) && d > TF(0)) ? d * log(mu / d) + (d - mu) : (mu > TF(0) && d == TF(0)) ? -mu : TF(0);
+        } else {
+          lp = (mu > TF(0) && d > TF(0)) ? d * logf(mu) - mu - (d * logf(d) - d)
+                                        : (mu > TF(0) && d == TF(0)) ? -mu : TF(0);
+        }
       } else {
-        G[Tr::IE1] += ax * S1;
-        H[Tr::IE1] += aax * S1 - ax * ax * S2;
-        G[Tr::IE2] += px * S8;
-        H[Tr::IE2] += px * S9 - px * px * S11;
+        // camera_models.jl:73-85
+        v = mu + (TF)src_var(src, cc, i);
+        const TF res = d - mu;
+        lp = v > TF(0) ? TF(-0.5) * (res * res / v) : TF(0);
+      }
+      if (cvalid) llr += lp;
+      if (cvalid && mu > TF(0)) {
+        const TF wv = TF(1) / v;
+        int q = 0;
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+          const TF ak = du[k] * wv;
+#pragma unroll
+          for (int l = k; l < P; ++l) Fu[q++] += ak * du[l];
+        }
       }
     }
+  };
+  if constexpr (Src::kInRegisters) {
+    // VGPR-resident pixels: the column index must be a compile-time constant.
+#pragma unroll
+    for (int cc = 0; cc < CB; ++cc) column(cc);
+  } else {
+    // Pixels from memory: one column at a time keeps only that column's loads
+    // in flight.
+#pragma unroll 1
+    for (int cc = 0; cc < CB; ++cc) column(cc);
   }
+  FinalOut<TF, P> out;
+  out.llr = group_sum<L>(llr);
+#pragma unroll
+  for (int q = 0; q < NF; ++q) Fu[q] = group_sum<L>(Fu[q]);
+  TF F[P * P];
+  {
+    int q = 0;
+#pragma unroll
+    for (int k = 0; k < P; ++k)
+#pragma unroll
+      for (int l = k; l < P; ++l) {
+        F[k + P * l] = Fu[q];
+        F[l + P * k] = Fu[q];
+        ++q;
+      }
+  }
+  invert_fisher<P, TF>(F, out.crlb);
+  return out;
+}
+
+// Out-of-range sCMOS corner: no fit, NaN outputs.
+template <class TF, int P> __device__ __forceinline__ void nan_outputs(float* th, FinalOut<TF, P>& fo) {
+  const float qnan = __builtin_nanf("");
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    th[k] = qnan;
+    fo.crlb[k] = TF(qnan);
+  }
+  fo.llr = TF(qnan);
+}
+
+template <int P, class TF>
+__device__ __forceinline__ void store_outputs(const FitArgs& a, int64_t roi, const float* th, const FinalOut<TF, P>& fo) {
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    a.params[roi * P + k] = th[k];
+    a.crlb[roi * P + k] = (float)fo.crlb[k];
+  }
+  a.llr[roi] = (float)fo.llr;
 }
 
 // ---------------------------------------------------------------------------
@@ -474,25 +776,6 @@ __global__ __launch_bounds__(kWave) void fit_kernel(FitArgs a) {
     const float nph = th[Tr::IN];
     const float bg = th[Tr::IBG];
 
-    AxisTab<float, N> ry;
-    axis_tables<float, N, SIG>(th[Tr::IY], w.sy, nph, 0, ry);
-    const AxisK<float> kc = axis_consts<float>(th[Tr::IX], w.sx, nph);
-
-    // Row extras: A = d(model)/d(extra) on the row axis, AA its second derivative.
-    float Ay[SIG ? N : 1], AAy[SIG ? N : 1];
-    if constexpr (SIG) {
-#pragma unroll
-      for (int i = 0; i < N; ++i) {
-        if constexpr (MODEL == ASTIG) {
-          Ay[i] = w.dsy * ry.S[i];
-          AAy[i] = ry.SS[i] * (w.dsy * w.dsy) + ry.S[i] * w.d2sy;
-        } else {
-          Ay[i] = ry.S[i];
-          AAy[i] = ry.SS[i];
-        }
-      }
-    }
-
     // Poisson fast path: every pixel has model > 0.01 and no cap can bind.
     // model >= bg - N*tiny for N <= 1e5 (phi >= 0 up to erf rounding), so
     // bg >= 0.05 keeps it above 0.01, and dmax <= 5000 bg min(bg,1) keeps
@@ -506,23 +789,52 @@ __global__ __launch_bounds__(kWave) void fit_kernel(FitArgs a) {
              (dmax <= 5000.f * bg * fminf(bg, 1.f));
     }
 
-    // Row sums for the q = cf + 1 form of the fast path.
-    float sphi = 0.f, sD = 0.f, sD2 = 0.f, sA = 0.f, sAA = 0.f;
-    if (fast) {
-#pragma unroll
-      for (int i = 0; i < N; ++i) {
-        sphi += ry.phi[i];
-        sD += ry.D[i];
-        sD2 += ry.D2[i];
-        if constexpr (SIG) { sA += Ay[i]; sAA += AAy[i]; }
-      }
-    }
-
     float G[P], H[P];
-    if (fast) {
-      accumulate<MODEL, CAM, N, L, true>(dv, vr, c0, nph, bg, ry, kc, w, Ay, AAy, sphi, sD, sD2, sA, sAA, G, H);
+    if constexpr (row_outer<MODEL, CAM, N, L>()) {
+      const AxisK<float> kr = axis_consts<float>(th[Tr::IY], w.sy, nph);
+      const AxisK<float> kc = axis_consts<float>(th[Tr::IX], w.sx, nph);
+      if (fast) {
+        accumulate_rows<MODEL, CAM, N, L, true>(dv, vr, c0, nph, bg, kr, kc, w, G, H);
+      } else {
+        accumulate_rows<MODEL, CAM, N, L, false>(dv, vr, c0, nph, bg, kr, kc, w, G, H);
+      }
     } else {
-      accumulate<MODEL, CAM, N, L, false>(dv, vr, c0, nph, bg, ry, kc, w, Ay, AAy, sphi, sD, sD2, sA, sAA, G, H);
+      AxisTab<float, N> ry;
+      axis_tables<float, N, SIG>(th[Tr::IY], w.sy, nph, 0, ry);
+      const AxisK<float> kc = axis_consts<float>(th[Tr::IX], w.sx, nph);
+
+      // Row extras: A = d(model)/d(extra) on the row axis, AA its second derivative.
+      float Ay[SIG ? N : 1], AAy[SIG ? N : 1];
+      if constexpr (SIG) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          if constexpr (MODEL == ASTIG) {
+            Ay[i] = w.dsy * ry.S[i];
+            AAy[i] = ry.SS[i] * (w.dsy * w.dsy) + ry.S[i] * w.d2sy;
+          } else {
+            Ay[i] = ry.S[i];
+            AAy[i] = ry.SS[i];
+          }
+        }
+      }
+
+      // Row sums for the q = cf + 1 form of the fast path.
+      float sphi = 0.f, sD = 0.f, sD2 = 0.f, sA = 0.f, sAA = 0.f;
+      if (fast) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          sphi += ry.phi[i];
+          sD += ry.D[i];
+          sD2 += ry.D2[i];
+          if constexpr (SIG) { sA += Ay[i]; sAA += AAy[i]; }
+        }
+      }
+
+      if (fast) {
+        accumulate<MODEL, CAM, N, L, true>(dv, vr, c0, nph, bg, ry, kc, w, Ay, AAy, sphi, sD, sD2, sA, sAA, G, H);
+      } else {
+        accumulate<MODEL, CAM, N, L, false>(dv, vr, c0, nph, bg, ry, kc, w, Ay, AAy, sphi, sD, sD2, sA, sAA, G, H);
+      }
     }
 #pragma unroll
     for (int k = 0; k < P; ++k) {
@@ -540,122 +852,59 @@ __global__ __launch_bounds__(kWave) void fit_kernel(FitArgs a) {
   }
 
   // ---- final pass: LLR + Fisher + CRLB (unified_kernel.jl:353-433) --------
-  using TF = typename std::conditional<PREC == PREC_MIXED, double, float>::type;
-  TF thf[P];
+  // Float32 here; the Float64 final pass runs as its own kernel
+  // (final_pass_kernel) so that its registers do not lower this kernel's
+  // occupancy.
+  if constexpr (PREC == PREC_FP32) {
+    const RegSrc<N, CAM == SCMOS> srcr{dv, vr};
+    FinalOut<float, P> fo = final_pass<MODEL, CAM, N, L, float>(th, a.psf, c0, srcr);
+    if (!inb) nan_outputs(th, fo);
+    if (active && r == 0) store_outputs<P>(a, roi, th, fo);
+  } else {
+    if (!inb) {
 #pragma unroll
-  for (int k = 0; k < P; ++k) thf[k] = (TF)th[k];
-  Widths<TF> wf;
-  model_widths<MODEL, TF>(thf, a.psf, wf);
-  const TF nphf = thf[Tr::IN], bgf = thf[Tr::IBG];
-  AxisTab<TF, N> fy;
-  axis_tables<TF, N, SIG>(thf[Tr::IY], wf.sy, nphf, 0, fy);
-  const AxisK<TF> kf = axis_consts<TF>(thf[Tr::IX], wf.sx, nphf);
+      for (int k = 0; k < P; ++k) th[k] = __builtin_nanf("");
+    }
+    if (active && r == 0) {
+#pragma unroll
+      for (int k = 0; k < P; ++k) a.params[roi * P + k] = th[k];
+    }
+  }
+}
 
-  constexpr int NF = P * (P + 1) / 2;
-  TF Fu[NF];
+// Float64 final pass at the estimates the iteration kernel wrote to params,
+// one lane per ROI; the ROI is re-read from HBM (once) instead of being held
+// in VGPRs.
+template <int MODEL, int CAM, int N>
+__global__ __launch_bounds__(kWave) void final_pass_kernel(FitArgs a) {
+  constexpr int P = MT<MODEL>::P;
+  const int64_t roi = (int64_t)blockIdx.x * kWave + threadIdx.x;
+  if (roi >= a.n_rois) return;
+  float th[P];
 #pragma unroll
-  for (int q = 0; q < NF; ++q) Fu[q] = TF(0);
-  TF llr = TF(0);
-  Edge<TF> flo = edge_eval<TF, false>(kf, c0);
-#pragma unroll
-  for (int cc = 0; cc < CB; ++cc) {
-    const Edge<TF> fhi = edge_eval<TF, false>(kf, c0 + cc + 1);
-    const Pix<TF> fx = pix_from_edges<TF, SIG>(kf, flo, fhi);
-    flo = fhi;
-    const TF px = fx.phi;
-    const bool cvalid = (L == 1) || (c0 + cc < N);
-    TF ax = TF(0);
-    if constexpr (SIG) ax = (MODEL == ASTIG) ? wf.dsx * fx.S : fx.S;
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-      const TF py = fy.phi[i];
-      const TF d = (TF)dv[cc * N + i];
-      const TF mu = bgf + nphf * px * py;
-      TF du[P];
-      du[Tr::IX] = fx.D * py;
-      du[Tr::IY] = fy.D[i] * px;
-      du[Tr::IN] = px * py;
-      du[Tr::IBG] = TF(1);
-      if constexpr (SIG) {
-        const TF ay = (MODEL == ASTIG) ? wf.dsy * fy.S[i] : fy.S[i];
-        if constexpr (Tr::EXTRA == 2) {
-          du[Tr::IE1] = ax * py;
-          du[Tr::IE2] = ay * px;
-        } else {
-          du[Tr::IE1] = ax * py + ay * px;
-        }
-      }
-      TF v = mu;
-      TF lp;
-      if constexpr (CAM == IDEAL) {
-        // camera_models.jl:60-71
-        if constexpr (PREC == PREC_MIXED) {
-          lp = (mu > TF(0) && d > TF(0)) ? d * log(mu / d) + (d - mu) : (mu > TF(0) && d == TF(0)) ? -mu : TF(0);
-        } else {
-          lp = (mu > TF(0) && d > TF(0)) ? d * logf(mu) - mu - (d * logf(d) - d)
-                                        : (mu > TF(0) && d == TF(0)) ? -mu : TF(0);
-        }
-      } else {
-        // camera_models.jl:73-85
-        v = mu + (TF)vr[cc * N + i];
-        const TF res = d - mu;
-        lp = v > TF(0) ? TF(-0.5) * (res * res / v) : TF(0);
-      }
-      if (cvalid) llr += lp;
-      if (cvalid && mu > TF(0)) {   // unified_kernel.jl:378-396
-        const TF wv = TF(1) / v;
-        int q = 0;
-#pragma unroll
-        for (int k = 0; k < P; ++k) {
-          const TF ak = du[k] * wv;
-#pragma unroll
-          for (int l = k; l < P; ++l) Fu[q++] += ak * du[l];
-        }
-      }
-    }
+  for (int k = 0; k < P; ++k) th[k] = a.params[roi * P + k];
+  GlobalSrc<N, CAM == SCMOS> src;
+  src.data = a.data + roi * (int64_t)(N * N);
+  bool inb = true;
+  if constexpr (CAM == SCMOS) {
+    const int cx = a.x_corners[roi], cy = a.y_corners[roi];
+    inb = (cx >= 1) && (cy >= 1) && (cx <= a.nx - N + 1) && (cy <= a.ny - N + 1);
+    src.var = a.var_map + (int64_t)(cy - 1) + (int64_t)a.ny * (int64_t)(cx - 1);
+    src.ny = a.ny;
+    if (!inb) src.var = nullptr;
   }
-  llr = group_sum<L>(llr);
-#pragma unroll
-  for (int q = 0; q < NF; ++q) Fu[q] = group_sum<L>(Fu[q]);
-  TF F[P * P];
-  {
-    int q = 0;
-#pragma unroll
-    for (int k = 0; k < P; ++k)
-#pragma unroll
-      for (int l = k; l < P; ++l) {
-        F[k + P * l] = Fu[q];
-        F[l + P * k] = Fu[q];
-        ++q;
-      }
+  FinalOut<double, P> fo;
+  if (inb) {
+    fo = final_pass<MODEL, CAM, N, 1, double>(th, a.psf, 0, src);
+  } else {
+    nan_outputs(th, fo);
   }
-  TF crlb[P];
-  invert_fisher<P, TF>(F, crlb);
-
-  if (!inb) {   // out-of-range sCMOS corner: no fit, NaN outputs
-    const float qnan = __builtin_nanf("");
+  store_outputs<P>(a, roi, th, fo);
+  if (a.crlb64) {
 #pragma unroll
-    for (int k = 0; k < P; ++k) {
-      th[k] = qnan;
-      crlb[k] = TF(qnan);
-    }
-    llr = TF(qnan);
+    for (int k = 0; k < P; ++k) a.crlb64[roi * P + k] = fo.crlb[k];
   }
-  if (active && r == 0) {
-#pragma unroll
-    for (int k = 0; k < P; ++k) {
-      a.params[roi * P + k] = th[k];
-      a.crlb[roi * P + k] = (float)crlb[k];
-    }
-    a.llr[roi] = (float)llr;
-    if constexpr (PREC == PREC_MIXED) {
-      if (a.crlb64) {
-#pragma unroll
-        for (int k = 0; k < P; ++k) a.crlb64[roi * P + k] = crlb[k];
-      }
-      if (a.llr64) a.llr64[roi] = llr;
-    }
-  }
+  if (a.llr64) a.llr64[roi] = fo.llr;
 }
 
 template <int MODEL, int CAM, int N, int L, int PREC>
@@ -664,6 +913,10 @@ hipError_t launch_fit(const FitArgs& a, hipStream_t stream) {
   const int64_t blocks = (a.n_rois + RPW - 1) / RPW;
   if (blocks <= 0) return hipSuccess;
   hipLaunchKernelGGL((fit_kernel<MODEL, CAM, N, L, PREC>), dim3((unsigned)blocks), dim3(kWave), 0, stream, a);
+  if constexpr (PREC == PREC_MIXED) {
+    const int64_t fblocks = (a.n_rois + kWave - 1) / kWave;
+    hipLaunchKernelGGL((final_pass_kernel<MODEL, CAM, N>), dim3((unsigned)fblocks), dim3(kWave), 0, stream, a);
+  }
   return hipGetLastError();
 }
 
