@@ -59,6 +59,94 @@ WORKLOADS = {
 }
 
 
+MIXED_LABEL = ("uneven batch: box sizes {7,9,11,13,15} (10/20/40/20/10%), models XYNB/XYNBS/XYNBSXSY uniform, "
+               "sCMOS 256x256 maps, 20 iterations")
+MODEL_KEYS = {0: "xynb", 1: "xynbs", 2: "xynbsxsy", 3: "astig"}
+
+
+def mixed_flops(mb) -> float:
+    """Total algorithmic FLOPs of one pass over a mixed batch."""
+    from gaussmle_amd import batch as gb
+    cam = "scmos" if mb.camera else "ideal"
+    tot = 0.0
+    for s, c in enumerate(mb.shape_counts()):
+        if c:
+            m, b = gb.shape_of(s)
+            tot += c * flops_per_fit(MODEL_KEYS[m], b, mb.iterations, cam)
+    return tot
+
+
+def mixed_bytes(mb) -> float:
+    P = np.array([4, 5, 6, 5])[mb.models.astype(np.int64)]
+    b = mb.boxes.astype(np.int64)
+    return float(np.sum(4 * b * b + 8 + 8 * P + 8))
+
+
+def run_mixed(n: int, steps: int, warmup: int, precision: str, tune_reps: int = 3, seed: int = 3):
+    """Config-3 style uneven batch: autotune, then time the batched call
+    against per-shape plain launches (each shape's ROIs contiguous, its best
+    lanes per ROI, launched one after another on one stream)."""
+    import torch
+    from gaussmle_amd import autotune
+    from gaussmle_amd import batch as gb
+
+    t0 = time.perf_counter()
+    mb = gb.config3(n=n, seed=seed)
+    print(f"[bench] generated mixed batch of {n} ROIs in {time.perf_counter() - t0:.1f}s", file=sys.stderr,
+          flush=True)
+    d = gb.upload_batch(mb)
+    stacks = autotune.shape_stacks(mb, d)
+    t0 = time.perf_counter()
+    tuning = autotune.tune(mb, d, precision, reps=tune_reps, stacks=stacks)
+    print(f"[bench] autotuned in {time.perf_counter() - t0:.1f}s: streams={tuning.streams} "
+          f"lanes={ {f'{k[0]}:{k[1]}': v for k, v in tuning.lanes.items()} }", file=sys.stderr, flush=True)
+    table = tuning.table()
+    out = {}
+    plain_outs = {}
+    best_alone = {k: min(v, key=v.get) for k, v in tuning.alone_ms.items()}
+
+    def batched(streams):
+        out.update(gb.fit_batch_device(mb, d, precision, table, streams, out=out or None))
+
+    def plain():
+        for key in stacks:
+            autotune.plain_launch(mb, d, stacks, key, best_alone[key], precision, plain_outs)
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        s = torch.cuda.current_stream()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+        t = time.perf_counter()
+        for a, b in ev:
+            a.record(s)
+            fn()
+            b.record(s)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t
+        return wall, float(np.mean([a.elapsed_time(b) for a, b in ev])) / 1e3
+
+    wall_b, dev_b = timed(lambda: batched(tuning.streams))
+    wall_1, dev_1 = timed(lambda: batched(1))
+    wall_p, dev_p = timed(plain)
+    res = {
+        "workload": MIXED_LABEL, "rois": n, "precision": precision,
+        "batched_fits_per_s": round(steps * n / wall_b, 1),
+        "batched_one_stream_fits_per_s": round(steps * n / wall_1, 1),
+        "per_shape_plain_fits_per_s": round(steps * n / wall_p, 1),
+        "batched_over_plain": round(wall_p / wall_b, 4),
+        "batched_ms": round(dev_b * 1e3, 4), "plain_ms": round(dev_p * 1e3, 4),
+        "streams": tuning.streams,
+        "lanes": {f"{k[0]}:{k[1]}": v for k, v in tuning.lanes.items()},
+        "plain_lanes": {f"{k[0]}:{k[1]}": v for k, v in best_alone.items()},
+        "shape_counts": {f"{gb.shape_of(int(s))[0]}:{gb.shape_of(int(s))[1]}": int(c)
+                         for s, c in enumerate(mb.shape_counts()) if c},
+        "flops": mixed_flops(mb), "bytes": mixed_bytes(mb),
+    }
+    return mb, res, dev_b
+
+
 def make_workload(spec, n):
     from gaussmle_amd import synth
     return getattr(synth, spec["gen"])(n=n)
@@ -109,7 +197,9 @@ def main(argv=None) -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="xynb11", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="xynb11", choices=sorted(WORKLOADS) + ["mixed"])
+    ap.add_argument("--mixed-rois", type=int, default=1_000_000,
+                    help="ROIs of the uneven batch reported beside the headline line (0: skip)")
     ap.add_argument("--rois", type=int, default=0, help="ROIs per step (default: workload's)")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "mixed"])
     ap.add_argument("--lanes", type=int, default=0)
@@ -130,6 +220,8 @@ def main(argv=None) -> int:
     from gaussmle_amd import _lib
     from gaussmle_amd import device as gd
 
+    if args.workload == "mixed":
+        return main_mixed(args)
     spec = WORKLOADS[args.workload]
     n = args.rois or spec["rois"]
     t0 = time.perf_counter()
@@ -202,6 +294,10 @@ def main(argv=None) -> int:
                          f"(oracle/refcpu.cpp, OpenMP), {dt:.1f}s wall",
                "single_thread_fits_per_s": round(rate1, 1)}
 
+    batched = None
+    if args.workload == "xynb11" and args.mixed_rois > 0:
+        _, batched, _ = run_mixed(args.mixed_rois, max(3, args.steps // 4), 2, args.precision)
+
     rec = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -221,9 +317,59 @@ def main(argv=None) -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "config0_batch": small or None,
+        "batched": batched,
     }
     print(json.dumps(rec), flush=True)
     return 0
+
+
+def main_mixed(args) -> int:
+    """--workload mixed: the uneven batch (4M ROIs by default) is the line."""
+    n = args.rois or 4_000_000
+    mb, res, dev_s = run_mixed(n, args.steps, args.warmup, args.precision)
+    achieved = res["flops"] / dev_s / 1e12
+    roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                "kernel_ms": round(dev_s * 1e3, 4), "flops_per_fit": round(res["flops"] / n, 1),
+                "hbm_gbs": round(res["bytes"] / dev_s / 1e9, 2),
+                "note": "whole batched call (grouping kernels + every shape's fit kernel), HIP events"}
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline_mixed(mb, max(20_000, args.cpu_sample // 2), min(os.cpu_count() or 1, 16))
+    rec = {
+        "metric": METRIC, "value": res["batched_fits_per_s"], "unit": "fits/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(n / res["batched_fits_per_s"] * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if args.precision == "fp32" else "f32+f64",
+        "data": "synthetic: seeded sCMOS ROIs (numpy PCG64), device-resident before timing",
+        "config": {"workload": MIXED_LABEL, "rois_per_step": n, "iterations": 20, "camera": "scmos",
+                   "precision": args.precision, "parallelism": "single GPU"},
+        "roofline": roofline, "cpu_baseline": cpu, "batched": res,
+    }
+    print(json.dumps(rec), flush=True)
+    return 0
+
+
+def cpu_baseline_mixed(mb, sample: int, threads: int):
+    """The oracle on the first `sample` ROIs of the batch, shape by shape."""
+    from gaussmle_amd import batch as gb
+    from gaussmle_amd import synth
+    from oracle import refcpu
+    refcpu.build()
+    sub = np.arange(min(sample, mb.n))
+    total = 0.0
+    for s, c in enumerate(mb.subset(sub).shape_counts()):
+        if not c:
+            continue
+        m, b = gb.shape_of(s)
+        idx = sub[(mb.models[sub] == m) & (mb.boxes[sub] == b)]
+        w = synth.Workload("s", mb.psfs[m], b, mb.roi_stack(idx, b), mb.truth[:, idx], mb.constraints[m],
+                           mb.iterations, mb.var_map, mb.x_corners[idx].copy(), mb.y_corners[idx].copy())
+        t0 = time.perf_counter()
+        refcpu.fit_workload(w, refcpu.FP32, nthreads=threads)
+        total += time.perf_counter() - t0
+    return {"value": round(len(sub) / total, 1), "unit": "fits/s", "cores": threads, "kind": "port",
+            "sample": f"first {len(sub)} ROIs of the same batch, Float32 oracle shape by shape, {total:.1f}s"}
 
 
 if __name__ == "__main__":

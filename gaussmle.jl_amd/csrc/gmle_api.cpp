@@ -11,12 +11,15 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "../../include/gaussmle.h"
 #include "gmle_args.hpp"
+#include "gmle_batch.hpp"
 
 namespace gmle {
 
@@ -132,11 +135,14 @@ void fill_args(FitArgs& a, int model, const float* psf, const float* lower, cons
 }
 
 // Device buffers cached across host-pointer calls (grow only).
+// Slots: 0 data, 1 params, 2 crlb, 3 llr, 4 var_map, 5 corners, 6 crlb64,
+// 7 llr64, 8 models, 9 boxes, 10 offsets, 11 batch workspace.
+constexpr int kCacheSlots = 12;
 struct Cache {
   std::mutex mu;
   int device = -1;
-  void* buf[8] = {};
-  size_t cap[8] = {};
+  void* buf[kCacheSlots] = {};
+  size_t cap[kCacheSlots] = {};
   hipStream_t stream = nullptr;
 };
 Cache g_cache;
@@ -149,6 +155,210 @@ hipError_t ensure(int slot, size_t bytes) {
   hipError_t e = hipMalloc(&g_cache.buf[slot], bytes);
   if (e == hipSuccess) g_cache.cap[slot] = bytes;
   return e;
+}
+
+// Points the cache at the current device (caller holds g_cache.mu).
+int cache_on_current_device() {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  if (g_cache.device != dev) {
+    // Buffers belong to another device: drop them (not freed there, since that
+    // device may not be current) and start over on this one.
+    for (int s = 0; s < kCacheSlots; ++s) { g_cache.buf[s] = nullptr; g_cache.cap[s] = 0; }
+    g_cache.stream = nullptr;
+    g_cache.device = dev;
+  }
+  if (!g_cache.stream) {
+    e = hipStreamCreateWithFlags(&g_cache.stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
+  }
+  return GMLE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Mixed-shape batches.
+// ---------------------------------------------------------------------------
+constexpr int kMaxStreams = 4;
+constexpr int kMaxDevices = 64;
+
+// Extra streams (default priority) and fork / join events per device; the
+// caller's stream is the first of the set.
+struct StreamSet {
+  std::mutex mu;
+  bool ready = false;
+  hipStream_t s[kMaxStreams - 1] = {};
+  hipEvent_t fork = nullptr;
+  hipEvent_t join[kMaxStreams - 1] = {};
+};
+StreamSet g_streams[kMaxDevices];
+
+int stream_set(StreamSet** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices) return fail(GMLE_ERR_UNSUPPORTED, "device %d beyond the stream table", dev);
+  StreamSet& ss = g_streams[dev];
+  if (!ss.ready) {
+    for (int k = 0; k < kMaxStreams - 1; ++k) {
+      if ((e = hipStreamCreateWithFlags(&ss.s[k], hipStreamNonBlocking)) != hipSuccess)
+        return hip_fail(e, "hipStreamCreate");
+      if ((e = hipEventCreateWithFlags(&ss.join[k], hipEventDisableTiming)) != hipSuccess)
+        return hip_fail(e, "hipEventCreate");
+    }
+    if ((e = hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming)) != hipSuccess)
+      return hip_fail(e, "hipEventCreate");
+    ss.ready = true;
+  }
+  *out = &ss;
+  return GMLE_OK;
+}
+
+// Relative cost of one ROI of a shape: FP32 operations per Newton iteration
+// of the separable formulation (per pixel plus per edge on both axes).
+double shape_cost(int model, int box) {
+  const double c_pix = model == XYNB ? 32 : model == XYNBSXSY ? 50 : 45;
+  const double c_axis = model == XYNB ? 13 : 23;
+  return box * box * c_pix + 2.0 * box * c_axis;
+}
+
+struct BatchSpec {
+  int64_t n;
+  const int8_t* models;
+  const int8_t* boxes;
+  const int64_t* offsets;
+  const float* data;
+  const float* psf;
+  const float* lower;
+  const float* upper;
+  const float* max_step;
+  int camera;
+  const float* var_map;
+  int ny, nx;
+  const int32_t* xc;
+  const int32_t* yc;
+  int iterations, precision;
+  const int32_t* lanes;
+  int streams;
+  const int64_t* counts;   // host, kNumShapes
+  float* params;
+  float* crlb;
+  float* llr;
+  double* crlb64;
+  double* llr64;
+  void* workspace;
+};
+
+// Checks shared by both batch entry points, once the shape counts are known.
+int check_batch(const BatchSpec& b) {
+  if (b.camera != IDEAL && b.camera != SCMOS) return fail(GMLE_ERR_ARG, "unknown camera model %d", b.camera);
+  if (b.precision != PREC_FP32 && b.precision != PREC_MIXED)
+    return fail(GMLE_ERR_ARG, "unknown precision %d", b.precision);
+  if (b.iterations < 0) return fail(GMLE_ERR_ARG, "iterations must be >= 0, got %d", b.iterations);
+  if (b.streams < 1 || b.streams > kMaxStreams)
+    return fail(GMLE_ERR_ARG, "streams must be 1..%d, got %d", kMaxStreams, b.streams);
+  if (!b.psf || !b.lower || !b.upper || !b.max_step)
+    return fail(GMLE_ERR_ARG, "psf and constraints arrays must not be NULL");
+  int64_t total = 0;
+  for (int s = 0; s < kNumShapes; ++s) {
+    if (b.counts[s] < 0) return fail(GMLE_ERR_ARG, "negative shape count");
+    total += b.counts[s];
+    if (b.counts[s] == 0) continue;
+    const int model = s / kNumBoxes, box = kMinBox + s % kNumBoxes;
+    const float* psf = b.psf + 8 * model;
+    if (model == XYNB && !(psf[0] > 0.f)) return fail(GMLE_ERR_ARG, "PSF sigma must be > 0");
+    if (model == ASTIG && !(psf[0] > 0.f && psf[1] > 0.f && psf[7] != 0.f && psf[6] != 0.f))
+      return fail(GMLE_ERR_ARG, "astigmatic PSF needs sigma_x0, sigma_y0 > 0 and nonzero gamma, d");
+    if (b.camera == SCMOS && (b.ny < box || b.nx < box))
+      return fail(GMLE_ERR_ARG, "variance map %dx%d is smaller than one ROI", b.ny, b.nx);
+    const int lanes = (b.lanes && b.lanes[s] > 0) ? b.lanes[s] : 0;
+    if (lanes && !lanes_ok(box, lanes))
+      return fail(GMLE_ERR_UNSUPPORTED, "no kernel with %d lanes per ROI for box size %d", lanes, box);
+  }
+  if (total > b.n) return fail(GMLE_ERR_ARG, "shape counts add up to %lld > n_rois %lld", (long long)total, (long long)b.n);
+  return GMLE_OK;
+}
+
+// Enqueues the grouping kernels on `stream`, then one fit launch per shape
+// present, spread over b.streams streams (largest estimated work first, each
+// to the least loaded stream), joined back into `stream`.
+int enqueue_batch(const BatchSpec& b, hipStream_t stream) {
+  BatchPlan p;
+  std::memset(&p, 0, sizeof(p));
+  p.models = b.models;
+  p.boxes = b.boxes;
+  p.n_rois = b.n;
+  char* ws = (char*)b.workspace;
+  p.order = (int32_t*)ws;
+  p.cursor = (int32_t*)(ws + align256((size_t)b.n * sizeof(int32_t)));
+  p.params = b.params;
+  p.crlb = b.crlb;
+  p.llr = b.llr;
+  p.crlb64 = b.precision == PREC_MIXED ? b.crlb64 : nullptr;
+  p.llr64 = b.precision == PREC_MIXED ? b.llr64 : nullptr;
+  int64_t slot = 0;
+  std::vector<int> shapes;
+  for (int s = 0; s < kNumShapes; ++s) {
+    p.start[s] = (int32_t)slot;
+    p.count[s] = (int32_t)b.counts[s];
+    slot += b.counts[s];
+    if (b.counts[s] > 0) shapes.push_back(s);
+  }
+  hipError_t e = enqueue_batch_plan(p, stream);
+  if (e != hipSuccess) return hip_fail(e, "batch planning launch");
+
+  std::sort(shapes.begin(), shapes.end(), [&](int u, int v) {
+    const double cu = b.counts[u] * shape_cost(u / kNumBoxes, kMinBox + u % kNumBoxes);
+    const double cv = b.counts[v] * shape_cost(v / kNumBoxes, kMinBox + v % kNumBoxes);
+    return cu > cv;
+  });
+  const int ns = std::min<int>(b.streams, std::max<int>(1, (int)shapes.size()));
+  StreamSet* ss = nullptr;
+  std::unique_lock<std::mutex> lock;
+  hipStream_t strm[kMaxStreams] = {stream};
+  if (ns > 1) {
+    int rc = stream_set(&ss);
+    if (rc != GMLE_OK) return rc;
+    lock = std::unique_lock<std::mutex>(ss->mu);
+    if ((e = hipEventRecord(ss->fork, stream)) != hipSuccess) return hip_fail(e, "hipEventRecord");
+    for (int k = 1; k < ns; ++k) {
+      strm[k] = ss->s[k - 1];
+      if ((e = hipStreamWaitEvent(strm[k], ss->fork, 0)) != hipSuccess) return hip_fail(e, "hipStreamWaitEvent");
+    }
+  }
+  double load[kMaxStreams] = {};
+  for (int s : shapes) {
+    const int model = s / kNumBoxes, box = kMinBox + s % kNumBoxes;
+    int lanes = (b.lanes && b.lanes[s] > 0) ? b.lanes[s] : default_lanes(model, b.camera, box, b.precision, b.counts[s]);
+    LaunchFn fn = find_launcher(model, b.camera, box, lanes, b.precision);
+    if (!fn) return fail(GMLE_ERR_UNSUPPORTED, "no kernel for model %d box %d lanes %d", model, box, lanes);
+    FitArgs a;
+    fill_args(a, model, b.psf + 8 * model, b.lower + 6 * model, b.upper + 6 * model, b.max_step + 6 * model,
+              b.counts[s], b.ny, b.nx, b.iterations);
+    a.data = b.data;
+    a.var_map = b.camera == SCMOS ? b.var_map : nullptr;
+    a.x_corners = b.camera == SCMOS ? b.xc : nullptr;
+    a.y_corners = b.camera == SCMOS ? b.yc : nullptr;
+    a.params = b.params;
+    a.crlb = b.crlb;
+    a.llr = b.llr;
+    a.crlb64 = p.crlb64;
+    a.llr64 = p.llr64;
+    a.order = p.order;
+    a.offsets = b.offsets;
+    a.first = p.start[s];
+    a.out_stride = kBatchStride;
+    int k = 0;
+    for (int q = 1; q < ns; ++q)
+      if (load[q] < load[k]) k = q;
+    load[k] += b.counts[s] * shape_cost(model, box);
+    if ((e = fn(a, strm[k])) != hipSuccess) return hip_fail(e, "kernel launch");
+  }
+  for (int k = 1; k < ns; ++k) {
+    if ((e = hipEventRecord(ss->join[k - 1], strm[k])) != hipSuccess) return hip_fail(e, "hipEventRecord");
+    if ((e = hipStreamWaitEvent(stream, ss->join[k - 1], 0)) != hipSuccess) return hip_fail(e, "hipStreamWaitEvent");
+  }
+  return GMLE_OK;
 }
 
 }  // namespace
@@ -281,20 +491,9 @@ int gmle_fit(int model, const float* psf, int box, int64_t n_rois, const float* 
   if (!fn) return fail(GMLE_ERR_UNSUPPORTED, "no kernel for model %d box %d lanes %d", model, box, lanes);
 
   std::lock_guard<std::mutex> lock(g_cache.mu);
-  int dev = 0;
-  hipError_t e = hipGetDevice(&dev);
-  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
-  if (g_cache.device != dev) {
-    // Buffers belong to another device: drop them (not freed there, since that
-    // device may not be current) and start over on this one.
-    for (int s = 0; s < 8; ++s) { g_cache.buf[s] = nullptr; g_cache.cap[s] = 0; }
-    g_cache.stream = nullptr;
-    g_cache.device = dev;
-  }
-  if (!g_cache.stream) {
-    e = hipStreamCreateWithFlags(&g_cache.stream, hipStreamNonBlocking);
-    if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
-  }
+  rc = cache_on_current_device();
+  if (rc != GMLE_OK) return rc;
+  hipError_t e = hipSuccess;
   const int P = n_params(model);
   const size_t nvar = camera == SCMOS ? (size_t)ny * (size_t)nx : 0;
   const bool want64 = precision == PREC_MIXED && (crlb64 || llr64);
@@ -342,6 +541,149 @@ int gmle_fit(int model, const float* psf, int box, int64_t n_rois, const float* 
     e = hipMemcpyAsync(crlb64, a.crlb64, (size_t)n_rois * P * sizeof(double), hipMemcpyDeviceToHost, s);
   if (e == hipSuccess && want64 && llr64)
     e = hipMemcpyAsync(llr64, a.llr64, (size_t)n_rois * sizeof(double), hipMemcpyDeviceToHost, s);
+  if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync (download)");
+  e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+  return GMLE_OK;
+}
+
+int gmle_shape_index(int model, int box) { return shape_index(model, box); }
+
+size_t gmle_batch_workspace_bytes(int64_t n_rois) { return batch_workspace(n_rois < 0 ? 0 : n_rois); }
+
+int gmle_fit_batch_device(int64_t n_rois, const int8_t* d_models, const int8_t* d_boxes,
+                          const int64_t* d_offsets, const float* d_data, const float* psf, const float* lower,
+                          const float* upper, const float* max_step, int camera, const float* d_var_map, int ny,
+                          int nx, const int32_t* d_x_corners, const int32_t* d_y_corners, int iterations,
+                          int precision, const int32_t* lanes, int streams, const int64_t* shape_counts,
+                          float* d_params, float* d_crlb, float* d_llr, double* d_crlb64, double* d_llr64,
+                          void* d_workspace, size_t workspace_bytes, void* stream) {
+  if (n_rois <= 0) return fail(GMLE_ERR_ARG, "Input data array is empty");
+  if (n_rois > INT32_MAX) return fail(GMLE_ERR_UNSUPPORTED, "batches hold at most %d ROIs", INT32_MAX);
+  if (!d_models || !d_boxes || !d_offsets || !d_data || !d_params || !d_crlb || !d_llr || !d_workspace)
+    return fail(GMLE_ERR_ARG, "device pointers must not be NULL");
+  if (workspace_bytes < batch_workspace(n_rois))
+    return fail(GMLE_ERR_ARG, "workspace of %zu bytes is smaller than the %zu needed", workspace_bytes,
+                batch_workspace(n_rois));
+  if (camera == SCMOS && (!d_var_map || !d_x_corners || !d_y_corners))
+    return fail(GMLE_ERR_ARG, "sCMOS fits need the variance map and ROI corners");
+  hipStream_t s = (hipStream_t)stream;
+  int64_t counts[kNumShapes];
+  if (shape_counts) {
+    for (int q = 0; q < kNumShapes; ++q) counts[q] = shape_counts[q];
+  } else {
+    int32_t* d_counts = (int32_t*)((char*)d_workspace + align256((size_t)n_rois * sizeof(int32_t)) +
+                                   align256(kNumShapes * sizeof(int32_t)));
+    int32_t h[kNumShapes];
+    hipError_t e = enqueue_batch_count(d_models, d_boxes, n_rois, d_counts, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(h, d_counts, sizeof(h), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(e, "batch shape count");
+    for (int q = 0; q < kNumShapes; ++q) counts[q] = h[q];
+  }
+  BatchSpec b{n_rois, d_models, d_boxes, d_offsets, d_data, psf, lower, upper, max_step, camera, d_var_map, ny, nx,
+              d_x_corners, d_y_corners, iterations, precision, lanes, streams, counts, d_params, d_crlb, d_llr,
+              d_crlb64, d_llr64, d_workspace};
+  int rc = check_batch(b);
+  if (rc != GMLE_OK) return rc;
+  return enqueue_batch(b, s);
+}
+
+int gmle_fit_batch(int64_t n_rois, const int8_t* models, const int8_t* boxes, const float* data,
+                   const float* psf, const float* lower, const float* upper, const float* max_step,
+                   int camera, const float* var_map, int ny, int nx, const int32_t* x_corners,
+                   const int32_t* y_corners, int iterations, int precision, const int32_t* lanes, int streams,
+                   float* params, float* crlb, float* llr, double* crlb64, double* llr64) {
+  if (n_rois <= 0) return fail(GMLE_ERR_ARG, "Input data array is empty");
+  if (n_rois > INT32_MAX) return fail(GMLE_ERR_UNSUPPORTED, "batches hold at most %d ROIs", INT32_MAX);
+  if (!models || !boxes || !data || !params || !crlb || !llr)
+    return fail(GMLE_ERR_ARG, "data / output pointers must not be NULL");
+  if (camera == SCMOS && (!var_map || !x_corners || !y_corners))
+    return fail(GMLE_ERR_ARG, "sCMOS fits need the variance map and ROI corners");
+  std::vector<int64_t> offsets((size_t)n_rois);
+  int64_t counts[kNumShapes] = {};
+  int64_t npix = 0;
+  for (int64_t k = 0; k < n_rois; ++k) {
+    const int m = models[k], box = boxes[k];
+    if (m < XYNB || m > ASTIG) return fail(GMLE_ERR_ARG, "ROI %lld: unknown PSF model %d", (long long)(k + 1), m);
+    if (box < kMinBox) return fail(GMLE_ERR_ARG, "ROI size must be at least 3x3, got %dx%d", box, box);
+    if (box > kMaxBox)
+      return fail(GMLE_ERR_UNSUPPORTED, "ROI size %d exceeds the supported maximum %d", box, kMaxBox);
+    if (camera == SCMOS) {
+      const int cx = x_corners[k], cy = y_corners[k];
+      if (cx < 1 || cy < 1 || cx > nx - box + 1 || cy > ny - box + 1)
+        return fail(GMLE_ERR_BOUNDS, "ROI %lld at corner (x=%d, y=%d) lies outside the %dx%d variance map",
+                    (long long)(k + 1), cx, cy, ny, nx);
+    }
+    offsets[(size_t)k] = npix;
+    npix += (int64_t)box * box;
+    ++counts[shape_index(m, box)];
+  }
+  for (int64_t q = 0; q < npix; ++q)
+    if (!std::isfinite(data[q])) return fail(GMLE_ERR_DATA, "Input data contains NaN or Inf values");
+  BatchSpec b{n_rois, nullptr, nullptr, nullptr, nullptr, psf, lower, upper, max_step, camera, nullptr, ny, nx,
+              nullptr, nullptr, iterations, precision, lanes, streams, counts, nullptr, nullptr, nullptr,
+              nullptr, nullptr, nullptr};
+  int rc = check_batch(b);
+  if (rc != GMLE_OK) return rc;
+  if (gmle_device_count() <= 0) return fail(GMLE_ERR_NODEVICE, "no HIP device available");
+
+  std::lock_guard<std::mutex> lock(g_cache.mu);
+  rc = cache_on_current_device();
+  if (rc != GMLE_OK) return rc;
+  const size_t n = (size_t)n_rois;
+  const size_t nvar = camera == SCMOS ? (size_t)ny * (size_t)nx : 0;
+  const bool want64 = precision == PREC_MIXED && (crlb64 || llr64);
+  hipError_t e = hipSuccess;
+  if ((e = ensure(0, (size_t)npix * sizeof(float))) != hipSuccess) return hip_fail(e, "hipMalloc data");
+  if ((e = ensure(1, n * kBatchStride * sizeof(float))) != hipSuccess) return hip_fail(e, "hipMalloc params");
+  if ((e = ensure(2, n * kBatchStride * sizeof(float))) != hipSuccess) return hip_fail(e, "hipMalloc crlb");
+  if ((e = ensure(3, n * sizeof(float))) != hipSuccess) return hip_fail(e, "hipMalloc llr");
+  if (camera == SCMOS) {
+    if ((e = ensure(4, nvar * sizeof(float))) != hipSuccess) return hip_fail(e, "hipMalloc var_map");
+    if ((e = ensure(5, n * 2 * sizeof(int32_t))) != hipSuccess) return hip_fail(e, "hipMalloc corners");
+  }
+  if (want64) {
+    if ((e = ensure(6, n * kBatchStride * sizeof(double))) != hipSuccess) return hip_fail(e, "hipMalloc crlb64");
+    if ((e = ensure(7, n * sizeof(double))) != hipSuccess) return hip_fail(e, "hipMalloc llr64");
+  }
+  if ((e = ensure(8, n)) != hipSuccess) return hip_fail(e, "hipMalloc models");
+  if ((e = ensure(9, n)) != hipSuccess) return hip_fail(e, "hipMalloc boxes");
+  if ((e = ensure(10, n * sizeof(int64_t))) != hipSuccess) return hip_fail(e, "hipMalloc offsets");
+  if ((e = ensure(11, batch_workspace(n_rois))) != hipSuccess) return hip_fail(e, "hipMalloc workspace");
+  hipStream_t s = g_cache.stream;
+  int32_t* d_corners = (int32_t*)g_cache.buf[5];
+  e = hipMemcpyAsync(g_cache.buf[0], data, (size_t)npix * sizeof(float), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(g_cache.buf[8], models, n, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(g_cache.buf[9], boxes, n, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(g_cache.buf[10], offsets.data(), n * sizeof(int64_t), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess && camera == SCMOS) {
+    e = hipMemcpyAsync(g_cache.buf[4], var_map, nvar * sizeof(float), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_corners, x_corners, n * sizeof(int32_t), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_corners + n, y_corners, n * sizeof(int32_t), hipMemcpyHostToDevice, s);
+  }
+  if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync (upload)");
+  b.models = (const int8_t*)g_cache.buf[8];
+  b.boxes = (const int8_t*)g_cache.buf[9];
+  b.offsets = (const int64_t*)g_cache.buf[10];
+  b.data = (const float*)g_cache.buf[0];
+  b.var_map = camera == SCMOS ? (const float*)g_cache.buf[4] : nullptr;
+  b.xc = camera == SCMOS ? d_corners : nullptr;
+  b.yc = camera == SCMOS ? d_corners + n : nullptr;
+  b.params = (float*)g_cache.buf[1];
+  b.crlb = (float*)g_cache.buf[2];
+  b.llr = (float*)g_cache.buf[3];
+  b.crlb64 = want64 ? (double*)g_cache.buf[6] : nullptr;
+  b.llr64 = want64 ? (double*)g_cache.buf[7] : nullptr;
+  b.workspace = g_cache.buf[11];
+  rc = enqueue_batch(b, s);
+  if (rc != GMLE_OK) return rc;
+  e = hipMemcpyAsync(params, b.params, n * kBatchStride * sizeof(float), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(crlb, b.crlb, n * kBatchStride * sizeof(float), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(llr, b.llr, n * sizeof(float), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess && want64 && crlb64)
+    e = hipMemcpyAsync(crlb64, b.crlb64, n * kBatchStride * sizeof(double), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess && want64 && llr64) e = hipMemcpyAsync(llr64, b.llr64, n * sizeof(double), hipMemcpyDeviceToHost, s);
   if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync (download)");
   e = hipStreamSynchronize(s);
   if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");

@@ -40,6 +40,16 @@ struct FitArgs {
   float lower[6];
   float upper[6];
   float max_step[6];
+  // Mixed-shape batches.  When `order` is set, this launch fits the ROIs
+  // order[first], ..., order[first + n_rois - 1] of a concatenated batch:
+  // ROI k's pixels start at data + offsets[k], its corner is
+  // (x_corners[k], y_corners[k]) and its results fill slot k of params / crlb
+  // (out_stride values per slot, NaN past P) and of llr.  Negative order
+  // entries are skipped.  Without `order`, ROI k sits at data + box*box*k.
+  const int32_t* order;
+  const int64_t* offsets;
+  int64_t first;
+  int out_stride;   // 0: P
 };
 
 // Host-side launcher of one (model, camera, box, lanes-per-ROI, precision)

@@ -641,43 +641,52 @@ template <class TF, int P> __device__ __forceinline__ void nan_outputs(float* th
 }
 
 template <int P, class TF>
-__device__ __forceinline__ void store_outputs(const FitArgs& a, int64_t roi, const float* th, const FinalOut<TF, P>& fo) {
+__device__ __forceinline__ void store_outputs(const FitArgs& a, int64_t out, int stride, const float* th,
+                                              const FinalOut<TF, P>& fo) {
 #pragma unroll
   for (int k = 0; k < P; ++k) {
-    a.params[roi * P + k] = th[k];
-    a.crlb[roi * P + k] = (float)fo.crlb[k];
+    a.params[out * stride + k] = th[k];
+    a.crlb[out * stride + k] = (float)fo.crlb[k];
   }
-  a.llr[roi] = (float)fo.llr;
+  a.llr[out] = (float)fo.llr;
+  for (int k = P; k < stride; ++k) {   // mixed-model batches: unused parameter slots
+    a.params[out * stride + k] = __builtin_nanf("");
+    a.crlb[out * stride + k] = __builtin_nanf("");
+  }
 }
 
 // ---------------------------------------------------------------------------
 // The kernel.
 // ---------------------------------------------------------------------------
+// Where one ROI's pixels live and where its results go.
+struct RoiRef {
+  const float* data;   // the ROI's first pixel (column-major N x N)
+  int64_t out;         // output slot
+  int stride;          // params / crlb values per slot
+  int cx, cy;          // 1-based camera corner (sCMOS)
+};
+
+// One ROI, fitted by the L lanes of its group (lane r of the group owns
+// columns [r*CB, r*CB + CB)).  Every lane of the wave must call this (the
+// cross-lane sums use DPP); `active` lanes store the results.
 template <int MODEL, int CAM, int N, int L, int PREC>
-__global__ __launch_bounds__(kWave) void fit_kernel(FitArgs a) {
+__device__ __forceinline__ void fit_one(const FitArgs& a, const RoiRef& ref, bool active, int r) {
   using Tr = MT<MODEL>;
   constexpr int P = Tr::P;
   constexpr int CB = (N + L - 1) / L;   // columns per lane
-  constexpr int RPW = kWave / L;        // ROIs per wave
   constexpr bool SIG = Tr::EXTRA != 0;  // sigma / z terms present
-
-  const int lane = threadIdx.x;
-  const int r = (L == 1) ? 0 : (lane & (L - 1));
-  const int64_t roi_raw = (int64_t)blockIdx.x * RPW + (lane / L);
-  const bool active = roi_raw < a.n_rois;
-  const int64_t roi = active ? roi_raw : (a.n_rois - 1);
   const int c0 = r * CB;
 
   // ---- load this lane's column block (and variances) once ----------------
   float dv[CB * N];
   float vr[CAM == SCMOS ? CB * N : 1];
-  const float* src = a.data + roi * (int64_t)(N * N);
+  const float* src = ref.data;
   float dmax_l = 0.f;
   int cxc = 1, cyc = 1;
   bool inb = true;   // sCMOS ROI inside the variance map
   if constexpr (CAM == SCMOS) {
-    cxc = a.x_corners[roi];
-    cyc = a.y_corners[roi];
+    cxc = ref.cx;
+    cyc = ref.cy;
     inb = (cxc >= 1) && (cyc >= 1) && (cxc <= a.nx - N + 1) && (cyc <= a.ny - N + 1);
   }
 #pragma unroll
@@ -859,7 +868,7 @@ __global__ __launch_bounds__(kWave) void fit_kernel(FitArgs a) {
     const RegSrc<N, CAM == SCMOS> srcr{dv, vr};
     FinalOut<float, P> fo = final_pass<MODEL, CAM, N, L, float>(th, a.psf, c0, srcr);
     if (!inb) nan_outputs(th, fo);
-    if (active && r == 0) store_outputs<P>(a, roi, th, fo);
+    if (active && r == 0) store_outputs<P>(a, ref.out, ref.stride, th, fo);
   } else {
     if (!inb) {
 #pragma unroll
@@ -867,8 +876,49 @@ __global__ __launch_bounds__(kWave) void fit_kernel(FitArgs a) {
     }
     if (active && r == 0) {
 #pragma unroll
-      for (int k = 0; k < P; ++k) a.params[roi * P + k] = th[k];
+      for (int k = 0; k < P; ++k) a.params[ref.out * ref.stride + k] = th[k];
     }
+  }
+}
+
+// The ROI that lane group `j` of a launch fits, and whether it stores.
+// Inactive groups (past the end, or skipped batch slots) fit a valid ROI of
+// the same shape and discard the result.
+template <int MODEL, int CAM, int N>
+__device__ __forceinline__ bool locate_roi(const FitArgs& a, int64_t j, int64_t fallback, RoiRef& ref) {
+  bool active = j < a.n_rois;
+  int64_t roi = active ? j : fallback;
+  int64_t off = roi * (int64_t)(N * N);
+  if (a.order) {
+    const int32_t k = a.order[a.first + roi];
+    active = active && (k >= 0);
+    roi = (k >= 0) ? k : a.order[a.first + fallback];
+    off = a.offsets[roi];
+  }
+  ref.data = a.data + off;
+  ref.out = roi;
+  ref.stride = a.out_stride > 0 ? a.out_stride : MT<MODEL>::P;
+  ref.cx = ref.cy = 1;
+  if constexpr (CAM == SCMOS) {
+    ref.cx = a.x_corners[roi];
+    ref.cy = a.y_corners[roi];
+  }
+  return active;
+}
+
+// One launch over n ROIs of one shape: 64/L ROIs per wave, in wave-sized
+// tiles (a grid smaller than the tile count strides over the rest).
+template <int MODEL, int CAM, int N, int L, int PREC>
+__global__ __launch_bounds__(kWave) void fit_kernel(FitArgs a) {
+  constexpr int RPW = kWave / L;
+  const int lane = threadIdx.x;
+  const int r = (L == 1) ? 0 : (lane & (L - 1));
+  // A batch slot range whose first entry is empty holds no ROI of this shape.
+  if (a.order && __builtin_amdgcn_readfirstlane(a.order[a.first]) < 0) return;
+  for (int64_t base = (int64_t)blockIdx.x * RPW; base < a.n_rois; base += (int64_t)gridDim.x * RPW) {
+    RoiRef ref;
+    const bool active = locate_roi<MODEL, CAM, N>(a, base + lane / L, 0, ref);
+    fit_one<MODEL, CAM, N, L, PREC>(a, ref, active, r);
   }
 }
 
@@ -878,16 +928,19 @@ __global__ __launch_bounds__(kWave) void fit_kernel(FitArgs a) {
 template <int MODEL, int CAM, int N>
 __global__ __launch_bounds__(kWave) void final_pass_kernel(FitArgs a) {
   constexpr int P = MT<MODEL>::P;
-  const int64_t roi = (int64_t)blockIdx.x * kWave + threadIdx.x;
-  if (roi >= a.n_rois) return;
+  if (a.order && __builtin_amdgcn_readfirstlane(a.order[a.first]) < 0) return;
+  RoiRef ref;
+  const bool active = locate_roi<MODEL, CAM, N>(a, (int64_t)blockIdx.x * kWave + threadIdx.x, 0, ref);
+  if (!active) return;
+  const int64_t slot = ref.out;
   float th[P];
 #pragma unroll
-  for (int k = 0; k < P; ++k) th[k] = a.params[roi * P + k];
+  for (int k = 0; k < P; ++k) th[k] = a.params[slot * ref.stride + k];
   GlobalSrc<N, CAM == SCMOS> src;
-  src.data = a.data + roi * (int64_t)(N * N);
+  src.data = ref.data;
   bool inb = true;
   if constexpr (CAM == SCMOS) {
-    const int cx = a.x_corners[roi], cy = a.y_corners[roi];
+    const int cx = ref.cx, cy = ref.cy;
     inb = (cx >= 1) && (cy >= 1) && (cx <= a.nx - N + 1) && (cy <= a.ny - N + 1);
     src.var = a.var_map + (int64_t)(cy - 1) + (int64_t)a.ny * (int64_t)(cx - 1);
     src.ny = a.ny;
@@ -899,12 +952,13 @@ __global__ __launch_bounds__(kWave) void final_pass_kernel(FitArgs a) {
   } else {
     nan_outputs(th, fo);
   }
-  store_outputs<P>(a, roi, th, fo);
+  store_outputs<P>(a, slot, ref.stride, th, fo);
   if (a.crlb64) {
 #pragma unroll
-    for (int k = 0; k < P; ++k) a.crlb64[roi * P + k] = fo.crlb[k];
+    for (int k = 0; k < P; ++k) a.crlb64[slot * ref.stride + k] = fo.crlb[k];
+    for (int k = P; k < ref.stride; ++k) a.crlb64[slot * ref.stride + k] = __builtin_nan("");
   }
-  if (a.llr64) a.llr64[roi] = fo.llr;
+  if (a.llr64) a.llr64[slot] = fo.llr;
 }
 
 template <int MODEL, int CAM, int N, int L, int PREC>

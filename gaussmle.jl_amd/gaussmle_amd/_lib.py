@@ -32,6 +32,8 @@ PRECISION_IDS = {"fp32": 0, "mixed": 1}
 _F = C.POINTER(C.c_float)
 _D = C.POINTER(C.c_double)
 _I32 = C.POINTER(C.c_int32)
+_I64 = C.POINTER(C.c_int64)
+_I8 = C.POINTER(C.c_int8)
 SIGNATURES = {
     "gmle_version": (C.c_char_p, []),
     "gmle_device_count": (C.c_int, []),
@@ -52,7 +54,23 @@ SIGNATURES = {
          C.c_void_p, _F, _F, _F, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
          C.c_void_p, C.c_void_p],
     ),
+    "gmle_shape_index": (C.c_int, [C.c_int, C.c_int]),
+    "gmle_batch_workspace_bytes": (C.c_size_t, [C.c_int64]),
+    "gmle_fit_batch": (
+        C.c_int,
+        [C.c_int64, _I8, _I8, _F, _F, _F, _F, _F, C.c_int, _F, C.c_int, C.c_int, _I32, _I32, C.c_int, C.c_int,
+         _I32, C.c_int, _F, _F, _F, _D, _D],
+    ),
+    "gmle_fit_batch_device": (
+        C.c_int,
+        [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, _F, _F, _F, _F, C.c_int, C.c_void_p, C.c_int,
+         C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int, _I32, C.c_int, _I64, C.c_void_p, C.c_void_p,
+         C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p],
+    ),
 }
+
+NUM_SHAPES = 56   # GMLE_NUM_SHAPES: 4 models x box sizes 3..16
+BATCH_STRIDE = 6  # params / crlb values per ROI slot of a mixed batch
 
 
 class GaussMLEError(RuntimeError):
@@ -133,6 +151,20 @@ def iptr(a: np.ndarray | None):
         return None
     assert a.dtype == np.int32 and a.flags.c_contiguous
     return a.ctypes.data_as(_I32)
+
+
+def i8ptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.dtype == np.int8 and a.flags.c_contiguous
+    return a.ctypes.data_as(_I8)
+
+
+def i64ptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.dtype == np.int64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_I64)
 
 
 def device_count() -> int:

@@ -127,15 +127,17 @@ def scmos_stack(rng, mu, var, gain, offset, xc, yc):
     """Electrons -> ADU -> electrons, as the camera and src/camera_models.jl:110-113 do."""
     box = mu.shape[0]
     ii = np.arange(box)
-    rows = (yc.astype(np.int64) - 1)[None, None, :] + ii[:, None, None]
-    cols = (xc.astype(np.int64) - 1)[None, None, :] + ii[None, :, None]
-    v = var[rows, cols].astype(np.float64)
-    g = gain[rows, cols].astype(np.float64)
-    o = offset[rows, cols].astype(np.float64)
-    e = rng.poisson(mu) + rng.standard_normal(mu.shape) * np.sqrt(v)
-    adu = (e / g + o).astype(F32)
+    nx = var.shape[1]
+    # flat C-order index of map element [row = yc + i - 1, col = xc + j - 1]
+    flat = (((yc.astype(np.int64) - 1)[None, None, :] + ii[:, None, None]) * nx
+            + (xc.astype(np.int64) - 1)[None, None, :] + ii[None, :, None])
+    v = np.take(var.ravel(), flat)
+    g = np.take(gain.ravel(), flat)
+    o = np.take(offset.ravel(), flat)
+    e = rng.poisson(mu) + rng.standard_normal(mu.shape) * np.sqrt(v.astype(np.float64))
+    adu = (e / g.astype(np.float64) + o.astype(np.float64)).astype(F32)
     out = np.empty(mu.shape, dtype=F32, order="F")
-    out[...] = (adu - o.astype(F32)) * g.astype(F32)
+    out[...] = (adu - o) * g
     return out
 
 

@@ -114,6 +114,62 @@ int gmle_fit_device(int model, const float* psf, int box, int64_t n_rois, const 
                     int iterations, int precision, int lanes, float* d_params, float* d_crlb, float* d_llr,
                     double* d_crlb64, double* d_llr64, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * Mixed-shape batches: ROIs of different box sizes and PSF models in one call.
+ *
+ * The reference fits one box size and one model per call (the kernel reads a
+ * single box size, src/unified_kernel.jl:301, and the model is a type
+ * parameter of the launch, :285-297), so a mixed set of ROIs takes one `fit`
+ * call (src/interface.jl:177-291) per (model, box size).  These entry points
+ * take the whole set at once:
+ *   models, boxes  Int8 (n_rois): ROI k's model (enum gmle_model) and box size.
+ *   data           ROI k's box*box Float32 pixels, column-major, start at
+ *                  data + offsets[k] (in floats).  The host entry point packs
+ *                  ROIs back to back in ROI order and needs no offsets.
+ *   psf            4 x 8 floats: row m is the psf argument of gmle_fit for
+ *                  model m (rows of models absent from the batch are ignored).
+ *   lower, upper, max_step   4 x 6 floats: row m holds model m's P values.
+ *   params, crlb   Float32 (6, n_rois): slot k holds ROI k's P values, then NaN.
+ *   llr            Float32 (n_rois).  crlb64 (6, n_rois) / llr64 (n_rois) as in
+ *                  gmle_fit (GMLE_MIXED only, optional).
+ *   lanes          GMLE_NUM_SHAPES ints indexed by gmle_shape_index(model, box):
+ *                  lanes per ROI of each shape (0 = automatic), or NULL.
+ *   streams        1-4: the per-shape launches are spread over this many HIP
+ *                  streams so that shapes overlap on the GPU (1 = in order).
+ * Every ROI's result is bit-identical to a gmle_fit call on that ROI alone with
+ * the same model, box, lanes and precision.
+ * ------------------------------------------------------------------------- */
+#define GMLE_NUM_SHAPES 56 /* 4 models x box sizes 3..16 */
+
+/* Index of (model, box) in the lanes / shape_counts tables, or -1. */
+int gmle_shape_index(int model, int box);
+
+/* Device workspace gmle_fit_batch_device needs for n_rois ROIs. */
+size_t gmle_batch_workspace_bytes(int64_t n_rois);
+
+/* Host arrays, synchronous.  data holds sum(boxes[k]^2) floats.  Validation
+ * as gmle_fit, per ROI (box sizes 3..16, known models, finite data, sCMOS
+ * ROIs inside var_map). */
+int gmle_fit_batch(int64_t n_rois, const int8_t* models, const int8_t* boxes, const float* data,
+                   const float* psf, const float* lower, const float* upper, const float* max_step,
+                   int camera, const float* var_map, int ny, int nx, const int32_t* x_corners,
+                   const int32_t* y_corners, int iterations, int precision, const int32_t* lanes, int streams,
+                   float* params, float* crlb, float* llr, double* crlb64, double* llr64);
+
+/* Device arrays (models, boxes, offsets, data, corners, outputs, workspace),
+ * enqueued on `stream`.  shape_counts (host, GMLE_NUM_SHAPES entries, or NULL)
+ * is the number of ROIs of each shape; with it the call returns without
+ * waiting, and ROIs beyond their shape's count get NaN outputs.  With NULL
+ * the library counts on the device and waits for the counts.  ROIs of an
+ * unknown model or box size get NaN outputs. */
+int gmle_fit_batch_device(int64_t n_rois, const int8_t* d_models, const int8_t* d_boxes,
+                          const int64_t* d_offsets, const float* d_data, const float* psf, const float* lower,
+                          const float* upper, const float* max_step, int camera, const float* d_var_map, int ny,
+                          int nx, const int32_t* d_x_corners, const int32_t* d_y_corners, int iterations,
+                          int precision, const int32_t* lanes, int streams, const int64_t* shape_counts,
+                          float* d_params, float* d_crlb, float* d_llr, double* d_crlb64, double* d_llr64,
+                          void* d_workspace, size_t workspace_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
