@@ -914,7 +914,7 @@ __global__ __launch_bounds__(kWave) void fit_kernel(FitArgs a) {
   const int lane = threadIdx.x;
   const int r = (L == 1) ? 0 : (lane & (L - 1));
   // A batch slot range whose first entry is empty holds no ROI of this shape.
-  if (a.order && __builtin_amdgcn_readfirstlane(a.order[a.first]) < 0) return;
+  if (a.n_rois <= 0 || (a.order && __builtin_amdgcn_readfirstlane(a.order[a.first]) < 0)) return;
   for (int64_t base = (int64_t)blockIdx.x * RPW; base < a.n_rois; base += (int64_t)gridDim.x * RPW) {
     RoiRef ref;
     const bool active = locate_roi<MODEL, CAM, N>(a, base + lane / L, 0, ref);
@@ -928,7 +928,7 @@ __global__ __launch_bounds__(kWave) void fit_kernel(FitArgs a) {
 template <int MODEL, int CAM, int N>
 __global__ __launch_bounds__(kWave) void final_pass_kernel(FitArgs a) {
   constexpr int P = MT<MODEL>::P;
-  if (a.order && __builtin_amdgcn_readfirstlane(a.order[a.first]) < 0) return;
+  if (a.n_rois <= 0 || (a.order && __builtin_amdgcn_readfirstlane(a.order[a.first]) < 0)) return;
   RoiRef ref;
   const bool active = locate_roi<MODEL, CAM, N>(a, (int64_t)blockIdx.x * kWave + threadIdx.x, 0, ref);
   if (!active) return;

@@ -10,7 +10,7 @@ counting, mixed precision, unknown shapes and short shape counts.
 import numpy as np
 import pytest
 
-from parity import assert_parity, compare, summary
+from parity import OUTLIER_FRACTION, compare, perturbed_workload, summary
 
 pytestmark = pytest.mark.gpu
 
@@ -126,19 +126,28 @@ def test_batch_against_oracle(name):
     from oracle import refcpu
     mb = mixed(name)
     r = gb.fit_batch(mb, "fp32", None, streams=4)
+    # The outlier rule of tests/parity.py applies to the batch as a whole
+    # (per shape, a few dozen ROIs would allow no outlier at all).
+    n_well = n_bad = n_fin = 0
     for (m, b) in resolved_lanes(mb, np.zeros(gb.NUM_SHAPES, np.int32), "fp32"):
-        idx = mb.roi_indices(m, b)[:400]
+        idx = mb.roi_indices(m, b)
         w = g.synth.Workload(f"{name}_{m}_{b}", mb.psfs[m], b, mb.roi_stack(idx, b), mb.truth[:, idx],
                              mb.constraints[m], mb.iterations, mb.var_map,
                              None if mb.x_corners is None else mb.x_corners[idx].copy(),
                              None if mb.y_corners is None else mb.y_corners[idx].copy())
         r32, r64 = refcpu.fit_workload(w, refcpu.FP32), refcpu.fit_workload(w, refcpu.FP64)
+        r64p = refcpu.fit_workload(perturbed_workload(w), refcpu.FP64)
         P = len(mb.psfs[m])
         res = {"params": r.params[:P, idx].T.astype(np.float64), "crlb": r.crlb[:P, idx].T.astype(np.float64),
                "llr": r.llr[idx].astype(np.float64)}
-        rep = compare(m, res, r32, r64, "fp32")
+        rep = compare(m, res, r32, r64, "fp32", r64p)
         print(f"[parity] batch {name} model={m} box={b} {summary(rep)}", flush=True)
-        assert_parity(rep, f"batch {name} model={m} box={b}")
+        n_well += rep["n"] - rep["n_ill"]
+        n_bad += rep["n_bad"]
+        n_fin += rep["finite_mismatch"]
+    print(f"[parity] batch {name} total well_posed={n_well} outliers={n_bad} finite_mismatch={n_fin}", flush=True)
+    assert n_bad <= int(OUTLIER_FRACTION * n_well)
+    assert n_fin == 0
 
 
 def test_batch_unknown_shapes_and_short_counts_give_nan():

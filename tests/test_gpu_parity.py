@@ -49,10 +49,16 @@ def workload(name):
 
 
 def reference(name, prec):
+    """The oracle's fit of a case; prec "fp64p" is the Float64 oracle on the
+    ulp-perturbed data (the conditioning check of tests/parity.py)."""
     from oracle import refcpu
+    from parity import perturbed_workload
     key = (name, prec)
     if key not in _REF:
-        _REF[key] = refcpu.fit_workload(workload(name), prec)
+        if prec == "fp64p":
+            _REF[key] = refcpu.fit_workload(perturbed_workload(workload(name)), refcpu.FP64)
+        else:
+            _REF[key] = refcpu.fit_workload(workload(name), prec)
     return _REF[key]
 
 
@@ -98,10 +104,10 @@ def _report(label, rep):
 def test_parity_host_path(name, precision):
     from oracle import refcpu
     w = workload(name)
-    r32, r64 = reference(name, refcpu.FP32), reference(name, refcpu.FP64)
+    r32, r64, r64p = reference(name, refcpu.FP32), reference(name, refcpu.FP64), reference(name, "fp64p")
     for lanes in lanes_for(w.box):
         print(f"[case] {name} {precision} lanes={lanes} n={w.n}", flush=True)
-        rep = compare(w.psf_pixels.model_id, gpu_fit_host(w, precision, lanes), r32, r64, precision)
+        rep = compare(w.psf_pixels.model_id, gpu_fit_host(w, precision, lanes), r32, r64, precision, r64p)
         _report(f"{name} {precision} L={lanes}", rep)
         assert_parity(rep, f"{name} {precision} L={lanes}")
 
@@ -110,14 +116,14 @@ def test_parity_host_path(name, precision):
 def test_parity_device_path(name):
     from oracle import refcpu
     w = workload(name)
-    r32, r64 = reference(name, refcpu.FP32), reference(name, refcpu.FP64)
+    r32, r64, r64p = reference(name, refcpu.FP32), reference(name, refcpu.FP64), reference(name, "fp64p")
     for lanes in lanes_for(w.box):
         print(f"[case] device {name} lanes={lanes}", flush=True)
         a = gpu_fit_device(w, "fp32", lanes)
         b = gpu_fit_host(w, "fp32", lanes)
         np.testing.assert_array_equal(a["params"], b["params"])   # same kernel, same bits
         np.testing.assert_array_equal(a["crlb"], b["crlb"])
-        rep = compare(w.psf_pixels.model_id, a, r32, r64, "fp32")
+        rep = compare(w.psf_pixels.model_id, a, r32, r64, "fp32", r64p)
         assert_parity(rep, f"device {name} L={lanes}")
 
 

@@ -48,3 +48,26 @@ def test_nan_where_reference_is_finite_fails(refs):
     rep = compare(0, g, r32, r64, "fp32")
     with pytest.raises(AssertionError):
         assert_parity(rep)
+
+
+def test_perturb_ulp_moves_every_nonzero_value_by_that_many_ulp():
+    from parity import perturb_ulp
+    d = np.float32([0.0, 1.0, -3.5, 1e-3, 250.0])
+    p = perturb_ulp(d, ulps=4, seed=1)
+    assert p[0] == 0.0
+    # same sign, so the distance in ulp is the distance of the bit patterns
+    steps = np.abs(p[1:].view(np.int32).astype(np.int64) - d[1:].view(np.int32).astype(np.int64))
+    np.testing.assert_array_equal(steps, 4)
+
+
+def test_ill_conditioned_rois_are_excluded(refs):
+    """An ROI whose perturbed Float64 fit moves beyond the tolerance is not
+    held to it; others still are."""
+    r32, r64 = refs
+    p = _copy(r64)
+    p["params"][7, 2] *= 1 + 5e-4          # the perturbed fit's N moved: ROI 7 is ill-conditioned
+    g = _copy(r32)
+    g["params"][7, 2] *= 1 + 5e-4
+    g["params"][9, 2] *= 1 + 5e-4
+    rep = compare(0, g, r32, r64, "fp32", p)
+    assert not rep["bad"][7] and rep["bad"][9] and rep["n_ill"] >= 1
