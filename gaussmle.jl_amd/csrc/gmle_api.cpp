@@ -20,10 +20,13 @@
 #include "../../include/gaussmle.h"
 #include "gmle_args.hpp"
 #include "gmle_batch.hpp"
+#include "gmle_host.hpp"
 
 namespace gmle {
 
-#define GMLE_DECLARE_FINDER(MODEL, CAM, PREC) LaunchFn find_launcher_##MODEL##_##CAM##_##PREC(int box, int lanes);
+#define GMLE_DECLARE_FINDER(MODEL, CAM, PREC)                             \
+  LaunchFn find_launcher_##MODEL##_##CAM##_##PREC(int box, int lanes); \
+  DescribeFn find_describe_##MODEL##_##CAM##_##PREC(int box, int lanes);
 #define GMLE_DECLARE_ALL(MODEL)               \
   GMLE_DECLARE_FINDER(MODEL, IDEAL, PREC_FP32) \
   GMLE_DECLARE_FINDER(MODEL, IDEAL, PREC_MIXED) \
@@ -52,9 +55,30 @@ LaunchFn find_launcher(int model, int camera, int box, int lanes, int precision)
   return nullptr;
 }
 
-namespace {
+bool find_kernels(int model, int camera, int box, int lanes, int precision, KernelInfo* out) {
+  DescribeFn fn = nullptr;
+#define GMLE_CASE(MODEL)                                                                   \
+  case MODEL:                                                                              \
+    if (camera == IDEAL)                                                                   \
+      fn = precision == PREC_FP32 ? find_describe_##MODEL##_IDEAL_PREC_FP32(box, lanes)    \
+                                  : find_describe_##MODEL##_IDEAL_PREC_MIXED(box, lanes);  \
+    else                                                                                   \
+      fn = precision == PREC_FP32 ? find_describe_##MODEL##_SCMOS_PREC_FP32(box, lanes)    \
+                                  : find_describe_##MODEL##_SCMOS_PREC_MIXED(box, lanes);  \
+    break;
+  switch (model) {
+    GMLE_CASE(XYNB)
+    GMLE_CASE(XYNBS)
+    GMLE_CASE(XYNBSXSY)
+    GMLE_CASE(ASTIG)
+  }
+#undef GMLE_CASE
+  return fn != nullptr && fn(out);
+}
 
+namespace {
 thread_local std::string g_err;
+}  // namespace
 
 int fail(int code, const char* fmt, ...) {
   char buf[512];
@@ -133,6 +157,8 @@ void fill_args(FitArgs& a, int model, const float* psf, const float* lower, cons
   a.nx = nx;
   a.iterations = iterations;
 }
+
+namespace {
 
 // Device buffers cached across host-pointer calls (grow only).
 // Slots: 0 data, 1 params, 2 crlb, 3 llr, 4 var_map, 5 corners, 6 crlb64,

@@ -59,4 +59,16 @@ typedef hipError_t (*LaunchFn)(const FitArgs&, hipStream_t);
 // Lookup implemented by the per-model translation units.
 LaunchFn find_launcher(int model, int camera, int box, int lanes, int precision);
 
+// The kernels behind a launcher, for callers that build HIP graphs: the fit
+// kernel (grid: ceil(n / rois_per_block) blocks of kWave threads) and, for
+// mixed precision, the Float64 final pass (ceil(n / kWave) blocks); both take
+// one FitArgs by value.
+struct KernelInfo {
+  const void* fit;
+  const void* final_pass;   // null for PREC_FP32
+  int rois_per_block;
+};
+typedef bool (*DescribeFn)(KernelInfo*);
+bool find_kernels(int model, int camera, int box, int lanes, int precision, KernelInfo* out);
+
 }  // namespace gmle

@@ -4,7 +4,7 @@
 namespace gmle {
 namespace {
 
-constexpr int kBlock = 256;
+constexpr int kBlock = kBatchBlock;
 
 __device__ void nan_slot(const BatchPlan& p, int64_t i) {
   const float qf = __builtin_nanf("");
@@ -74,6 +74,9 @@ __global__ __launch_bounds__(kBlock) void batch_count(const int8_t* models, cons
 }
 
 }  // namespace
+
+const void* batch_init_kernel() { return reinterpret_cast<const void*>(&batch_init); }
+const void* batch_scatter_kernel() { return reinterpret_cast<const void*>(&batch_scatter); }
 
 hipError_t enqueue_batch_plan(const BatchPlan& p, hipStream_t stream) {
   if (p.n_rois <= 0) return hipSuccess;

@@ -48,6 +48,13 @@ inline size_t batch_workspace(int64_t n) {
   return align256((size_t)n * sizeof(int32_t)) + 2 * align256(kNumShapes * sizeof(int32_t));
 }
 
+constexpr int kBatchBlock = 256;   // threads per block of the grouping kernels
+
+// The grouping kernels themselves (one BatchPlan by value; grid
+// ceil(n_rois / kBatchBlock) blocks of kBatchBlock threads), for HIP graphs.
+const void* batch_init_kernel();
+const void* batch_scatter_kernel();
+
 // Enqueue batch_init + batch_scatter.
 hipError_t enqueue_batch_plan(const BatchPlan& p, hipStream_t stream);
 // Enqueue batch_count into counts[kNumShapes] (device).

@@ -962,6 +962,14 @@ __global__ __launch_bounds__(kWave) void final_pass_kernel(FitArgs a) {
 }
 
 template <int MODEL, int CAM, int N, int L, int PREC>
+bool describe_fit(KernelInfo* k) {
+  k->fit = reinterpret_cast<const void*>(&fit_kernel<MODEL, CAM, N, L, PREC>);
+  k->final_pass = PREC == PREC_MIXED ? reinterpret_cast<const void*>(&final_pass_kernel<MODEL, CAM, N>) : nullptr;
+  k->rois_per_block = kWave / L;
+  return true;
+}
+
+template <int MODEL, int CAM, int N, int L, int PREC>
 hipError_t launch_fit(const FitArgs& a, hipStream_t stream) {
   constexpr int RPW = kWave / L;
   const int64_t blocks = (a.n_rois + RPW - 1) / RPW;

@@ -15,33 +15,44 @@ constexpr bool lanes_built(int box, int lanes) {
          ((lanes == 1 && box <= 12) || lanes == 2 || lanes == 4);
 }
 
-template <int MODEL, int CAM, int PREC, int N>
-LaunchFn pick_lanes(int lanes) {
-  if constexpr (lanes_built(N, 1)) {
-    if (lanes == 1) return &launch_fit<MODEL, CAM, N, 1, PREC>;
+// The launcher (Describe = false) or the kernel description (true) of one
+// specialisation, as a function pointer.
+template <int MODEL, int CAM, int PREC, int N, int L, bool Describe>
+constexpr auto entry() {
+  if constexpr (Describe) {
+    return &describe_fit<MODEL, CAM, N, L, PREC>;
+  } else {
+    return &launch_fit<MODEL, CAM, N, L, PREC>;
   }
-  if (lanes == 2) return &launch_fit<MODEL, CAM, N, 2, PREC>;
-  if (lanes == 4) return &launch_fit<MODEL, CAM, N, 4, PREC>;
+}
+
+template <int MODEL, int CAM, int PREC, int N, bool Describe, class Fn>
+Fn pick_lanes(int lanes) {
+  if constexpr (lanes_built(N, 1)) {
+    if (lanes == 1) return entry<MODEL, CAM, PREC, N, 1, Describe>();
+  }
+  if (lanes == 2) return entry<MODEL, CAM, PREC, N, 2, Describe>();
+  if (lanes == 4) return entry<MODEL, CAM, PREC, N, 4, Describe>();
   return nullptr;
 }
 
-template <int MODEL, int CAM, int PREC>
-LaunchFn pick(int box, int lanes) {
+template <int MODEL, int CAM, int PREC, bool Describe = false, class Fn = LaunchFn>
+Fn pick(int box, int lanes) {
   switch (box) {
-    case 3: return pick_lanes<MODEL, CAM, PREC, 3>(lanes);
-    case 4: return pick_lanes<MODEL, CAM, PREC, 4>(lanes);
-    case 5: return pick_lanes<MODEL, CAM, PREC, 5>(lanes);
-    case 6: return pick_lanes<MODEL, CAM, PREC, 6>(lanes);
-    case 7: return pick_lanes<MODEL, CAM, PREC, 7>(lanes);
-    case 8: return pick_lanes<MODEL, CAM, PREC, 8>(lanes);
-    case 9: return pick_lanes<MODEL, CAM, PREC, 9>(lanes);
-    case 10: return pick_lanes<MODEL, CAM, PREC, 10>(lanes);
-    case 11: return pick_lanes<MODEL, CAM, PREC, 11>(lanes);
-    case 12: return pick_lanes<MODEL, CAM, PREC, 12>(lanes);
-    case 13: return pick_lanes<MODEL, CAM, PREC, 13>(lanes);
-    case 14: return pick_lanes<MODEL, CAM, PREC, 14>(lanes);
-    case 15: return pick_lanes<MODEL, CAM, PREC, 15>(lanes);
-    case 16: return pick_lanes<MODEL, CAM, PREC, 16>(lanes);
+    case 3: return pick_lanes<MODEL, CAM, PREC, 3, Describe, Fn>(lanes);
+    case 4: return pick_lanes<MODEL, CAM, PREC, 4, Describe, Fn>(lanes);
+    case 5: return pick_lanes<MODEL, CAM, PREC, 5, Describe, Fn>(lanes);
+    case 6: return pick_lanes<MODEL, CAM, PREC, 6, Describe, Fn>(lanes);
+    case 7: return pick_lanes<MODEL, CAM, PREC, 7, Describe, Fn>(lanes);
+    case 8: return pick_lanes<MODEL, CAM, PREC, 8, Describe, Fn>(lanes);
+    case 9: return pick_lanes<MODEL, CAM, PREC, 9, Describe, Fn>(lanes);
+    case 10: return pick_lanes<MODEL, CAM, PREC, 10, Describe, Fn>(lanes);
+    case 11: return pick_lanes<MODEL, CAM, PREC, 11, Describe, Fn>(lanes);
+    case 12: return pick_lanes<MODEL, CAM, PREC, 12, Describe, Fn>(lanes);
+    case 13: return pick_lanes<MODEL, CAM, PREC, 13, Describe, Fn>(lanes);
+    case 14: return pick_lanes<MODEL, CAM, PREC, 14, Describe, Fn>(lanes);
+    case 15: return pick_lanes<MODEL, CAM, PREC, 15, Describe, Fn>(lanes);
+    case 16: return pick_lanes<MODEL, CAM, PREC, 16, Describe, Fn>(lanes);
   }
   return nullptr;
 }
@@ -49,10 +60,14 @@ LaunchFn pick(int box, int lanes) {
 }  // namespace gmle
 
 // One translation unit per (model, camera, precision) defines
-//   LaunchFn find_launcher_<MODEL>_<CAM>_<PREC>(int box, int lanes)
+//   LaunchFn   find_launcher_<MODEL>_<CAM>_<PREC>(int box, int lanes)
+//   DescribeFn find_describe_<MODEL>_<CAM>_<PREC>(int box, int lanes)
 #define GMLE_DEFINE_FINDER(MODEL, CAM, PREC)                                     \
   namespace gmle {                                                               \
   LaunchFn find_launcher_##MODEL##_##CAM##_##PREC(int box, int lanes) {          \
     return pick<MODEL, CAM, PREC>(box, lanes);                                   \
+  }                                                                              \
+  DescribeFn find_describe_##MODEL##_##CAM##_##PREC(int box, int lanes) {        \
+    return pick<MODEL, CAM, PREC, true, DescribeFn>(box, lanes);                 \
   }                                                                              \
   }

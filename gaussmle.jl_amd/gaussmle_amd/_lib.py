@@ -69,6 +69,31 @@ SIGNATURES = {
     ),
 }
 
+
+
+class Request(C.Structure):
+    """struct gmle_request (include/gaussmle.h)."""
+    _fields_ = [("n_rois", C.c_int64), ("models", C.c_void_p), ("boxes", C.c_void_p), ("x_corners", C.c_void_p),
+                ("y_corners", C.c_void_p), ("adu", C.c_void_p), ("params", C.c_void_p), ("crlb", C.c_void_p),
+                ("llr", C.c_void_p)]
+
+
+_RP = C.POINTER(Request)
+_DP = C.POINTER(C.c_double)
+SIGNATURES.update({
+    "gmle_host_alloc": (C.c_void_p, [C.c_size_t]),
+    "gmle_host_free": (C.c_int, [C.c_void_p]),
+    "gmle_service_create": (C.c_int, [C.c_int, C.c_size_t, C.c_int, C.c_int, C.c_int, _F, _F, _F, _F, _I32,
+                                      C.POINTER(C.c_void_p)]),
+    "gmle_service_calibrate": (C.c_int, [C.c_void_p, _F, _F, _F, C.c_int, C.c_int]),
+    "gmle_service_submit": (C.c_int64, [C.c_void_p, _RP]),
+    "gmle_service_wait": (C.c_int, [C.c_void_p, C.c_int64, C.c_double]),
+    "gmle_service_replay": (C.c_int, [C.c_void_p, C.c_int64, _RP, _DP, _DP, _DP]),
+    "gmle_service_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int),
+                                    C.POINTER(C.c_int64)]),
+    "gmle_service_destroy": (C.c_int, [C.c_void_p]),
+})
+
 NUM_SHAPES = 56   # GMLE_NUM_SHAPES: 4 models x box sizes 3..16
 BATCH_STRIDE = 6  # params / crlb values per ROI slot of a mixed batch
 

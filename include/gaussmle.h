@@ -170,6 +170,81 @@ int gmle_fit_batch_device(int64_t n_rois, const int8_t* d_models, const int8_t* 
                           float* d_params, float* d_crlb, float* d_llr, double* d_crlb64, double* d_llr64,
                           void* d_workspace, size_t workspace_bytes, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * Serving: a resident fitting service with continuous batching.
+ *
+ * The reference is a library call per batch: upload, launch, wait, download
+ * (src/interface.jl:220-262, GPU branch) with the sCMOS variance map copied
+ * again for every batch (:232-233) and ADU -> electrons done on the host
+ * (src/camera_models.jl:110-113, src/interface.jl:391-392).  A service keeps
+ * the sensor's offset / gain / variance maps and per-slot device arenas
+ * resident, and takes fit requests (one camera frame's ROIs in raw uint16
+ * counts) as they arrive:
+ *   - a dispatcher thread packs every pending request that fits into a free
+ *     slot's arena (admission by the HBM budget fixed at creation) and
+ *     launches the slot's HIP graph: ingest (reads the requests' pinned
+ *     buffers directly, converts ADU to electrons with the resident maps),
+ *     grouping by shape, one fit kernel per served shape, result scatter
+ *     (writes straight into each request's pinned output buffers);
+ *   - up to `slots` batches are in flight at once, each on its own stream, so
+ *     requests arriving while others run are admitted as slots free up and
+ *     their work overlaps on the GPU;
+ *   - a request's results are bit-identical whichever requests share its
+ *     batch (each ROI is fitted alone, by its shape's kernel).
+ * Request buffers must come from gmle_host_alloc (pinned, device-visible) and
+ * stay untouched until the request completes.
+ * ------------------------------------------------------------------------- */
+typedef struct gmle_service gmle_service;
+
+typedef struct gmle_request {
+  int64_t n_rois;
+  const int8_t* models;       /* (n_rois) enum gmle_model */
+  const int8_t* boxes;        /* (n_rois) box sizes */
+  const int32_t* x_corners;   /* (n_rois) 1-based sensor column of each ROI's first pixel */
+  const int32_t* y_corners;   /* (n_rois) 1-based sensor row */
+  const uint16_t* adu;        /* raw counts, ROIs back to back, each box*box column-major */
+  float* params;              /* out (6, n_rois), as gmle_fit_batch */
+  float* crlb;                /* out (6, n_rois) */
+  float* llr;                 /* out (n_rois) */
+} gmle_request;
+
+/* Pinned, device-visible host memory for request buffers (NULL on failure). */
+void* gmle_host_alloc(size_t bytes);
+int gmle_host_free(void* p);
+
+/* A service on `device`.  hbm_budget: device bytes it may hold (maps plus
+ * `slots` arenas of equal size); psf / lower / upper / max_step as
+ * gmle_fit_batch; lanes (GMLE_NUM_SHAPES): lanes per ROI of each served shape,
+ * 0 for shapes the service does not serve. */
+int gmle_service_create(int device, size_t hbm_budget, int slots, int precision, int iterations,
+                        const float* psf, const float* lower, const float* upper, const float* max_step,
+                        const int32_t* lanes, gmle_service** out);
+
+/* Upload the sensor's maps, Float32 (ny, nx) column-major [row = y, col = x]:
+ * electrons = (adu - offset) * gain; var = read-noise variance (electrons^2). */
+int gmle_service_calibrate(gmle_service* s, const float* offset, const float* gain, const float* var, int ny,
+                           int nx);
+
+/* Queue a request; returns a ticket (>= 0) or a negative error code.  Checks:
+ * pinned buffers, served shapes, ROIs inside the sensor, size within one
+ * slot's capacity. */
+int64_t gmle_service_submit(gmle_service* s, const gmle_request* r);
+
+/* Wait for a ticket: 0 done (outputs written), 1 timed out, < 0 error. */
+int gmle_service_wait(gmle_service* s, int64_t ticket, double timeout_s);
+
+/* Benchmark driver: submits reqs[i] at arrival_s[i] seconds after the call
+ * starts (ascending), waits for all, and returns each request's latency from
+ * its scheduled arrival to completion, and the total elapsed time. */
+int gmle_service_replay(gmle_service* s, int64_t n, const gmle_request* reqs, const double* arrival_s,
+                        double* latency_s, double* elapsed_s);
+
+/* Capacity of one slot (ROIs, pixels), number of slots, batches launched. */
+int gmle_service_info(gmle_service* s, int64_t* slot_rois, int64_t* slot_pixels, int* slots, int64_t* batches);
+
+/* Waits for in-flight work, then frees everything. */
+int gmle_service_destroy(gmle_service* s);
+
 #ifdef __cplusplus
 }
 #endif
