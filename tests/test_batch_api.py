@@ -116,3 +116,51 @@ def test_mixed_workload_layout():
     b1 = int(mb.boxes[1])
     np.testing.assert_array_equal(sub.data[sub.offsets[1]:sub.offsets[1] + b1 * b1],
                                   mb.data[mb.offsets[1]:mb.offsets[1] + b1 * b1])
+
+
+def test_service_entry_points_without_device(hiplib):
+    """Argument checks of the service entry points that need no GPU."""
+    import ctypes as C
+    z = np.zeros(32, np.float32)
+    c = np.zeros(24, np.float32)
+    lanes = np.zeros(_lib.NUM_SHAPES, np.int32)
+    h = C.c_void_p()
+    f = _lib.fptr
+    # no shape served
+    assert hiplib.gmle_service_create(0, 1 << 30, 2, 0, 20, f(z), f(c), f(c), f(c), _lib.iptr(lanes),
+                                      C.byref(h)) == _lib.GMLE_ERR_ARG
+    lanes[gb.shape_index(0, 11)] = 3                 # no 3-lane kernel
+    assert hiplib.gmle_service_create(0, 1 << 30, 2, 0, 20, f(z), f(c), f(c), f(c), _lib.iptr(lanes),
+                                      C.byref(h)) == _lib.GMLE_ERR_UNSUPPORTED
+    lanes[gb.shape_index(0, 11)] = 2
+    assert hiplib.gmle_service_create(0, 1 << 30, 0, 0, 20, f(z), f(c), f(c), f(c), _lib.iptr(lanes),
+                                      C.byref(h)) == _lib.GMLE_ERR_ARG        # slots
+    assert hiplib.gmle_service_create(0, 1 << 30, 2, 0, 20, f(z), f(c), f(c), f(c), _lib.iptr(lanes),
+                                      C.byref(h)) == _lib.GMLE_ERR_ARG        # XYNB sigma 0
+    z[0] = 1.3
+    if _lib.device_count() == 0:
+        assert hiplib.gmle_service_create(0, 1 << 30, 2, 0, 20, f(z), f(c), f(c), f(c), _lib.iptr(lanes),
+                                          C.byref(h)) == _lib.GMLE_ERR_NODEVICE
+    assert hiplib.gmle_service_destroy(None) == 0
+    assert hiplib.gmle_host_free(None) == 0
+
+
+def test_frames_generator_layout():
+    from gaussmle_amd import service as sv
+    maps = sv.sensor_maps(seed=3, ny=128, nx=160)
+    frs = sv.make_frames(6, seed=4, maps=maps, rois=(50, 400))
+    for f in frs:
+        assert 50 <= f.n <= 400
+        assert f.adu.dtype == np.uint16 and f.adu.size == int((f.boxes.astype(int) ** 2).sum())
+        assert set(np.unique(f.boxes)) <= {9, 11, 13, 15} and set(np.unique(f.models)) <= {0, 1}
+        assert np.all(f.xc >= 1) and np.all(f.xc + f.boxes - 1 <= 160)
+        assert np.all(f.yc >= 1) and np.all(f.yc + f.boxes - 1 <= 128)
+    # electrons follow the expected image: offset 100 ADU, gain ~0.5
+    e = sv.electrons(frs[0], maps)
+    assert abs(float(np.median(e))) < 100
+    mb = sv.frames_batch(frs, maps, *sv.default_setup()[:2])
+    assert mb.n == sum(f.n for f in frs)
+    idx = mb.roi_indices(0, 11)
+    from gaussmle_amd import synth
+    mu = synth.expected_counts(mb.psfs[0], mb.truth[:4, idx], 11)
+    assert np.corrcoef(mb.roi_stack(idx, 11).ravel(), mu.ravel())[0, 1] > 0.95
