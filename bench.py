@@ -147,6 +147,67 @@ def run_mixed(n: int, steps: int, warmup: int, precision: str, tune_reps: int = 
     return mb, res, dev_b
 
 
+def run_service(n_frames: int = 300, submissions: int = 3000, loads=(0.1, 0.5, 0.9, 1.0), slots: int = 4,
+                lanes=None, seed: int = 4):
+    """Streaming service: seeded frames (ROI count log-uniform 50-5000, box
+    sizes {9,11,13,15}, XYNB / XYNBS) of a 2048x2048 uint16 sCMOS sensor;
+    Poisson arrivals at fractions of the service's measured peak; p50 / p99
+    latency from scheduled arrival to completion, and sustained fits/s."""
+    from gaussmle_amd import batch as gb
+    from gaussmle_amd import service as sv
+    import torch
+
+    t0 = time.perf_counter()
+    maps = sv.sensor_maps(seed=seed, ny=2048, nx=2048)
+    frames = sv.make_frames(n_frames, seed=seed + 1, maps=maps)
+    rois = np.array([f.n for f in frames])
+    print(f"[bench] generated {n_frames} frames ({rois.sum()} ROIs) in {time.perf_counter() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    psfs, cons, table = sv.default_setup(lanes=lanes if lanes is not None else 2)
+    reqs = [sv.Request(f) for f in frames]
+    rng = np.random.default_rng(seed + 2)
+    res = {"workload": "frames of 50-5000 ROIs (log-uniform), box 9-15, XYNB/XYNBS, 2048x2048 uint16 sCMOS",
+           "frames": n_frames, "mean_rois_per_frame": round(float(rois.mean()), 1), "slots": slots,
+           "lanes": {f"{m}:{b}": L for (m, b), L in table.items()}}
+    with sv.Service(psfs, cons, table, hbm_budget=8 << 30, slots=slots) as svc:
+        svc.calibrate(*maps)
+        lat, el = svc.replay(reqs, np.zeros(len(reqs)))          # warm-up
+        peaks = []
+        for _ in range(3):
+            lat, el = svc.replay(reqs, np.zeros(len(reqs)))
+            peaks.append(rois.sum() / el)
+        peak = max(peaks)
+        res["peak_fits_per_s"] = round(peak, 1)
+        res["loads"] = []
+        for frac in loads:
+            order = rng.integers(0, len(reqs), submissions)
+            rate = frac * peak / rois.mean()                         # requests / s
+            arrivals = np.cumsum(rng.exponential(1.0 / rate, submissions))
+            arrivals -= arrivals[0]
+            lat, el = svc.replay([reqs[i] for i in order], arrivals)
+            res["loads"].append({"load": frac, "offered_fits_per_s": round(float(rate * rois.mean()), 1),
+                                 "sustained_fits_per_s": round(float(rois[order].sum() / el), 1),
+                                 "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 4),
+                                 "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 4),
+                                 "max_ms": round(float(lat.max()) * 1e3, 4)})
+            print(f"[bench] service load {frac}: {res['loads'][-1]}", file=sys.stderr, flush=True)
+        res["batches"] = svc.info()["batches"]
+    for r in reqs:
+        r.close()
+    # the plain batch kernel on the same ROIs (electrons, device-resident)
+    mb = sv.frames_batch(frames, maps, psfs, cons)
+    d = gb.upload_batch(mb)
+    t = gb._lanes_table(table)
+    out = gb.fit_batch_device(mb, d, "fp32", t, 4)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(5):
+        out = gb.fit_batch_device(mb, d, "fp32", t, 4, out=out)
+    torch.cuda.synchronize()
+    res["plain_batch_fits_per_s"] = round(5 * mb.n / (time.perf_counter() - t1), 1)
+    return res
+
+
 def make_workload(spec, n):
     from gaussmle_amd import synth
     return getattr(synth, spec["gen"])(n=n)
@@ -200,6 +261,8 @@ def main(argv=None) -> int:
     ap.add_argument("--workload", default="xynb11", choices=sorted(WORKLOADS) + ["mixed"])
     ap.add_argument("--mixed-rois", type=int, default=1_000_000,
                     help="ROIs of the uneven batch reported beside the headline line (0: skip)")
+    ap.add_argument("--serve-frames", type=int, default=300,
+                    help="frames of the streaming-service run reported beside the headline line (0: skip)")
     ap.add_argument("--rois", type=int, default=0, help="ROIs per step (default: workload's)")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "mixed"])
     ap.add_argument("--lanes", type=int, default=0)
@@ -297,6 +360,9 @@ def main(argv=None) -> int:
     batched = None
     if args.workload == "xynb11" and args.mixed_rois > 0:
         _, batched, _ = run_mixed(args.mixed_rois, max(3, args.steps // 4), 2, args.precision)
+    serving = None
+    if args.workload == "xynb11" and args.serve_frames > 0:
+        serving = run_service(args.serve_frames)
 
     rec = {
         "metric": METRIC,
@@ -318,6 +384,7 @@ def main(argv=None) -> int:
         "cpu_baseline": cpu,
         "config0_batch": small or None,
         "batched": batched,
+        "serving": serving,
     }
     print(json.dumps(rec), flush=True)
     return 0
