@@ -71,4 +71,100 @@ function hip_fit!(results::Matrix{Float32}, uncertainties::Matrix{Float32}, log_
     return results, uncertainties, log_likelihoods
 end
 
+function check(rc)
+    if rc != GMLE_OK
+        msg = last_error()
+        rc in (-1, -2, -3) ? throw(ArgumentError(msg)) : error("gaussmle-hip: " * msg)
+    end
+    return nothing
+end
+
+"""
+    hip_fit_batch!(params, crlb, llr, models, boxes, data, psf, lower, upper, max_step,
+                   scmos, variance_map, x_corners, y_corners, iterations;
+                   precision=:fp32, lanes=nothing, streams=4)
+
+A mixed set of ROIs in one call: `models`/`boxes` are Vector{Int8}(n) (model
+ids as `model_id`, box sizes 3..16); `data` holds the ROIs back to back, each
+box*box Float32 column-major; `psf` is Matrix{Float32}(8, 4) and the
+constraints Matrix{Float32}(6, 4), column m+1 for model id m; outputs are
+Matrix{Float32}(6, n) (each ROI's P values, then NaN) and Vector{Float32}(n).
+Replaces one `fit` call per (model, box size) on such a set.
+"""
+function hip_fit_batch!(params::Matrix{Float32}, crlb::Matrix{Float32}, llr::Vector{Float32},
+                        models::Vector{Int8}, boxes::Vector{Int8}, data::Vector{Float32},
+                        psf::Matrix{Float32}, lower::Matrix{Float32}, upper::Matrix{Float32},
+                        max_step::Matrix{Float32}, scmos::Bool, variance_map::Matrix{Float32},
+                        x_corners::Vector{Int32}, y_corners::Vector{Int32}, iterations::Integer;
+                        precision::Symbol = :fp32, lanes::Union{Nothing,Vector{Int32}} = nothing,
+                        streams::Integer = 4)
+    n = length(models)
+    ny, nx = size(variance_map)
+    lp = lanes === nothing ? Ptr{Int32}(C_NULL) : pointer(lanes)
+    rc = GC.@preserve params crlb llr models boxes data psf lower upper max_step variance_map x_corners y_corners lanes begin
+        ccall((:gmle_fit_batch, libgmle), Cint,
+              (Int64, Ptr{Int8}, Ptr{Int8}, Ptr{Float32}, Ptr{Float32}, Ptr{Float32}, Ptr{Float32}, Ptr{Float32},
+               Cint, Ptr{Float32}, Cint, Cint, Ptr{Int32}, Ptr{Int32}, Cint, Cint, Ptr{Int32}, Cint,
+               Ptr{Float32}, Ptr{Float32}, Ptr{Float32}, Ptr{Float64}, Ptr{Float64}),
+              n, models, boxes, data, psf, lower, upper, max_step, scmos ? 1 : 0, variance_map, ny, nx,
+              x_corners, y_corners, iterations, PRECISION[precision], lp, streams, params, crlb, llr,
+              Ptr{Float64}(C_NULL), Ptr{Float64}(C_NULL))
+    end
+    check(rc)
+    return params, crlb, llr
+end
+
+# ---------------------------------------------------------------------------
+# Serving (gmle_service_*): requests live in pinned memory from hip_host_alloc.
+# ---------------------------------------------------------------------------
+struct Request
+    n_rois::Int64
+    models::Ptr{Int8}
+    boxes::Ptr{Int8}
+    x_corners::Ptr{Int32}
+    y_corners::Ptr{Int32}
+    adu::Ptr{UInt16}
+    params::Ptr{Float32}
+    crlb::Ptr{Float32}
+    llr::Ptr{Float32}
+end
+
+"Pinned, device-visible host memory as a Vector{T} (free with hip_host_free)."
+function hip_host_alloc(::Type{T}, n::Integer) where {T}
+    p = ccall((:gmle_host_alloc, libgmle), Ptr{Cvoid}, (Csize_t,), n * sizeof(T))
+    p == C_NULL && error("gaussmle-hip: " * last_error())
+    return unsafe_wrap(Vector{T}, Ptr{T}(p), n; own = false)
+end
+hip_host_free(v::Vector) = check(ccall((:gmle_host_free, libgmle), Cint, (Ptr{Cvoid},), pointer(v)))
+
+function hip_service(psf::Matrix{Float32}, lower::Matrix{Float32}, upper::Matrix{Float32},
+                     max_step::Matrix{Float32}, lanes::Vector{Int32}; device = 0, hbm_budget = 4 << 30,
+                     slots = 4, precision::Symbol = :fp32, iterations = 20)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:gmle_service_create, libgmle), Cint,
+                (Cint, Csize_t, Cint, Cint, Cint, Ptr{Float32}, Ptr{Float32}, Ptr{Float32}, Ptr{Float32},
+                 Ptr{Int32}, Ptr{Ptr{Cvoid}}),
+                device, hbm_budget, slots, PRECISION[precision], iterations, psf, lower, upper, max_step, lanes, h))
+    return h[]
+end
+
+hip_service_calibrate(s, offset::Matrix{Float32}, gain::Matrix{Float32}, var::Matrix{Float32}) =
+    check(ccall((:gmle_service_calibrate, libgmle), Cint,
+                (Ptr{Cvoid}, Ptr{Float32}, Ptr{Float32}, Ptr{Float32}, Cint, Cint),
+                s, offset, gain, var, size(var, 1), size(var, 2)))
+
+function hip_service_submit(s, r::Request)
+    t = ccall((:gmle_service_submit, libgmle), Int64, (Ptr{Cvoid}, Ref{Request}), s, r)
+    t < 0 && check(Cint(t))
+    return t
+end
+
+function hip_service_wait(s, ticket::Integer; timeout = 60.0)
+    rc = ccall((:gmle_service_wait, libgmle), Cint, (Ptr{Cvoid}, Int64, Cdouble), s, ticket, timeout)
+    rc == 1 && error("gaussmle-hip: request $ticket not done after $(timeout)s")
+    return check(rc)
+end
+
+hip_service_destroy(s) = check(ccall((:gmle_service_destroy, libgmle), Cint, (Ptr{Cvoid},), s))
+
 end # module
