@@ -372,11 +372,12 @@ __device__ __forceinline__ void accumulate_rows(const float* dv, const float* vr
       S[cc][5] -= (float)N;
       if constexpr (SIG) { S[cc][7] -= sA; S[cc][8] -= sAA; }
     }
+    // Columns past the ROI (last lane of a group when L does not divide N)
+    // contribute nothing.  Selects, not a branch: a per-lane branch here
+    // splits the block and costs ~100 VGPRs of live ranges.
     const bool cvalid = (L == 1) || (c0 + cc < N);
-    if (!cvalid) {
 #pragma unroll
-      for (int q = 0; q < 11; ++q) S[cc][q] = 0.f;
-    }
+    for (int q = 0; q < 11; ++q) S[cc][q] = cvalid ? S[cc][q] : 0.f;
     fold_column<MODEL>(cx[cc], w, S[cc], G, H);
   }
   (void)NS;
@@ -478,10 +479,10 @@ __device__ __forceinline__ void accumulate(const float* dv, const float* vr, int
       S6 -= (float)N;
       if constexpr (SIG) { S8 -= sA; S9 -= sAA; }
     }
-    if (!cvalid) {
-      S1 = S2 = S3 = S4 = S5 = S6 = S7 = S8 = S9 = S10 = S11 = 0.f;
-    }
-    const float Sv[11] = {S1, S2, S3, S4, S5, S6, S7, S8, S9, S10, S11};
+    // Columns past the ROI contribute nothing (selects, not a branch).
+    const float Sv[11] = {cvalid ? S1 : 0.f, cvalid ? S2 : 0.f, cvalid ? S3 : 0.f, cvalid ? S4 : 0.f,
+                          cvalid ? S5 : 0.f, cvalid ? S6 : 0.f, cvalid ? S7 : 0.f, cvalid ? S8 : 0.f,
+                          cvalid ? S9 : 0.f, cvalid ? S10 : 0.f, cvalid ? S11 : 0.f};
     fold_column<MODEL>(cx, w, Sv, G, H);
   }
 }
