@@ -190,8 +190,9 @@ def run_service(n_frames: int = 300, submissions: int = 3000, loads=(0.1, 0.5, 0
                                  "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 4),
                                  "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 4),
                                  "max_ms": round(float(lat.max()) * 1e3, 4)})
-            print(f"[bench] service load {frac}: {res['loads'][-1]}", file=sys.stderr, flush=True)
+            print(f"[bench] service load {frac}: {res['loads'][-1]} {svc.stats()}", file=sys.stderr, flush=True)
         res["batches"] = svc.info()["batches"]
+        res["batch_stats"] = svc.stats()
     for r in reqs:
         r.close()
     # the plain batch kernel on the same ROIs (electrons, device-resident)
@@ -258,7 +259,9 @@ def main(argv=None) -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="xynb11", choices=sorted(WORKLOADS) + ["mixed"])
+    ap.add_argument("--workload", default="xynb11", choices=sorted(WORKLOADS) + ["mixed", "serve"])
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-lanes", type=int, default=2)
     ap.add_argument("--mixed-rois", type=int, default=1_000_000,
                     help="ROIs of the uneven batch reported beside the headline line (0: skip)")
     ap.add_argument("--serve-frames", type=int, default=300,
@@ -285,6 +288,10 @@ def main(argv=None) -> int:
 
     if args.workload == "mixed":
         return main_mixed(args)
+    if args.workload == "serve":
+        res = run_service(args.serve_frames or 300, slots=args.serve_slots, lanes=args.serve_lanes)
+        print(json.dumps({"metric": METRIC, "serving": res}), flush=True)
+        return 0
     spec = WORKLOADS[args.workload]
     n = args.rois or spec["rois"]
     t0 = time.perf_counter()

@@ -225,6 +225,7 @@ struct ShapeNode {
 struct Slot {
   int index = 0;
   hipStream_t stream = nullptr;
+  hipEvent_t start = nullptr;   // recorded before the graph (timing)
   hipEvent_t done = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -255,6 +256,8 @@ struct Slot {
   std::vector<PendingP> batch;
   bool busy = false;       // holds a batch (being launched or in flight)
   bool launched = false;   // its graph and completion event are enqueued
+  double t_formed = 0, t_launched = 0;   // host clock: batch taken, graph launched
+  int64_t batch_rois = 0;
   std::thread waiter;
 };
 
@@ -285,6 +288,8 @@ struct gmle_service {
   int64_t batches = 0;
   bool stop = false;
   std::thread dispatcher;
+  // per-batch statistics (sums)
+  double stat_batches = 0, stat_rois = 0, stat_reqs = 0, stat_host = 0, stat_gpu = 0, stat_seen = 0;
   int launch_error = GMLE_OK;
 };
 
@@ -399,7 +404,8 @@ int build_slot(gmle_service* s, Slot& sl, size_t slot_bytes) {
   sl.chunks_h = (Chunk*)h;
   sl.chunks_d = (Chunk*)d;
   if ((e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking)) != hipSuccess) return hip_fail(e, "hipStreamCreate");
-  if ((e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming)) != hipSuccess) return hip_fail(e, "hipEventCreate");
+  if ((e = hipEventCreate(&sl.start)) != hipSuccess) return hip_fail(e, "hipEventCreate");
+  if ((e = hipEventCreate(&sl.done)) != hipSuccess) return hip_fail(e, "hipEventCreate");
 
   // The graph: ingest -> init -> scatter -> fits (-> final passes) -> results.
   if ((e = hipGraphCreate(&sl.graph, 0)) != hipSuccess) return hip_fail(e, "hipGraphCreate");
@@ -498,6 +504,7 @@ int launch_slot(gmle_service* s, Slot& sl) {
         return hip_fail(e, "hipGraphExecKernelNodeSetParams");
     }
   }
+  if ((e = hipEventRecord(sl.start, sl.stream)) != hipSuccess) return hip_fail(e, "hipEventRecord");
   if ((e = hipGraphLaunch(sl.exec, sl.stream)) != hipSuccess) return hip_fail(e, "hipGraphLaunch");
   if ((e = hipEventRecord(sl.done, sl.stream)) != hipSuccess) return hip_fail(e, "hipEventRecord");
   return GMLE_OK;
@@ -538,9 +545,12 @@ void dispatcher_main(gmle_service* s) {
       s->pending.pop_front();
     }
     sl.busy = true;
+    sl.t_formed = now_s();
+    sl.batch_rois = rois;
     ++s->batches;
     lock.unlock();
     const int rc = launch_slot(s, sl);
+    sl.t_launched = now_s();
     lock.lock();
     if (rc != GMLE_OK) {
       s->launch_error = rc;
@@ -563,10 +573,22 @@ void waiter_main(gmle_service* s, Slot* sl) {
     s->cv_dispatch.wait(lock, [&] { return s->stop || sl->launched; });
     if (!sl->launched) return;   // stopping, nothing in flight
     lock.unlock();
-    const hipError_t e = hipEventSynchronize(sl->done);
+    // Poll rather than block: a blocking wait's wake-up costs more than a
+    // small batch takes on the GPU.
+    hipError_t e;
+    while ((e = hipEventQuery(sl->done)) == hipErrorNotReady) std::this_thread::yield();
+    const double t_seen = now_s();
+    float gpu_ms = 0.f;
+    if (e == hipSuccess) (void)hipEventElapsedTime(&gpu_ms, sl->start, sl->done);
     std::vector<PendingP> b;
     lock.lock();
     b.swap(sl->batch);
+    s->stat_batches += 1;
+    s->stat_rois += (double)sl->batch_rois;
+    s->stat_reqs += (double)b.size();
+    s->stat_host += sl->t_launched - sl->t_formed;
+    s->stat_gpu += gpu_ms * 1e-3;
+    s->stat_seen += t_seen - sl->t_launched;
     lock.unlock();
     finish(s, b, e == hipSuccess ? GMLE_OK : GMLE_ERR_HIP);
     lock.lock();
@@ -656,6 +678,7 @@ void teardown(gmle_service* s) {
     if (sl->exec) (void)hipGraphExecDestroy(sl->exec);
     if (sl->graph) (void)hipGraphDestroy(sl->graph);
     if (sl->done) (void)hipEventDestroy(sl->done);
+    if (sl->start) (void)hipEventDestroy(sl->start);
     if (sl->stream) (void)hipStreamDestroy(sl->stream);
     if (sl->reqs_h) (void)hipHostFree(sl->reqs_h);
     if (sl->chunks_h) (void)hipHostFree(sl->chunks_h);
@@ -861,6 +884,16 @@ int gmle_service_info(gmle_service* s, int64_t* slot_rois, int64_t* slot_pixels,
   if (slot_pixels) *slot_pixels = s->cap_pix;
   if (slots) *slots = (int)s->slots.size();
   if (batches) *batches = s->batches;
+  return GMLE_OK;
+}
+
+int gmle_service_stats(gmle_service* s, double* out, int n) {
+  if (!s || !out) return fail(GMLE_ERR_ARG, "NULL argument");
+  std::lock_guard<std::mutex> lock(s->mu);
+  const double b = s->stat_batches > 0 ? s->stat_batches : 1.0;
+  const double v[6] = {s->stat_batches, s->stat_rois / b, s->stat_reqs / b, s->stat_host / b * 1e6,
+                       s->stat_gpu / b * 1e6, s->stat_seen / b * 1e6};
+  for (int k = 0; k < n && k < 6; ++k) out[k] = v[k];
   return GMLE_OK;
 }
 

@@ -91,6 +91,7 @@ SIGNATURES.update({
     "gmle_service_replay": (C.c_int, [C.c_void_p, C.c_int64, _RP, _DP, _DP, _DP]),
     "gmle_service_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int),
                                     C.POINTER(C.c_int64)]),
+    "gmle_service_stats": (C.c_int, [C.c_void_p, _DP, C.c_int]),
     "gmle_service_destroy": (C.c_int, [C.c_void_p]),
 })
 

@@ -242,6 +242,11 @@ int gmle_service_replay(gmle_service* s, int64_t n, const gmle_request* reqs, co
 /* Capacity of one slot (ROIs, pixels), number of slots, batches launched. */
 int gmle_service_info(gmle_service* s, int64_t* slot_rois, int64_t* slot_pixels, int* slots, int64_t* batches);
 
+/* Per-batch averages since creation, up to n of: batches, ROIs per batch,
+ * requests per batch, host time to form and launch a batch (us), GPU time of
+ * a batch (us), time from launch to the completion being seen (us). */
+int gmle_service_stats(gmle_service* s, double* out, int n);
+
 /* Waits for in-flight work, then frees everything. */
 int gmle_service_destroy(gmle_service* s);
 

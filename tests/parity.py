@@ -21,7 +21,10 @@ itself moves by more than the tolerances when every pixel value moves by one
 Float32 ulp, so no Float32 implementation can be held to them.  An ROI is
 well-posed when
   (1) the Float32 oracle meets every tolerance against the Float64 oracle, and
-  (2) when given, the Float64 oracle on the data perturbed by a few ulp per
+  (2) in mixed precision, when given, the oracle's own mixed mode meets the
+      LLR tolerance against the Float64 oracle (the LLR is evaluated at the
+      Float32 iterations' estimate);
+  (3) when given, the Float64 oracle on the data perturbed by a few ulp per
       pixel (perturb_ulp: 4 ulp, about the rounding of a Float32 sum over a
       hundred pixel terms, sqrt(121)/2 ulp) meets every tolerance against the
       unperturbed Float64 oracle.
@@ -95,15 +98,24 @@ def perturbed_workload(w):
                           w.truth, w.constraints, w.iterations, w.var_map, w.x_corners, w.y_corners)
 
 
-def compare(model_id: int, gpu: dict, ref32: dict, ref64: dict, precision: str, ref64p: dict | None = None) -> dict:
+def compare(model_id: int, gpu: dict, ref32: dict, ref64: dict, precision: str, ref64p: dict | None = None,
+            refmix: dict | None = None) -> dict:
     """gpu/ref*: dict(params (n,P), crlb (n,P), llr (n,)); ref64p: the Float64
-    oracle on perturb_ulp(data), or None.  Returns a report."""
+    oracle on perturb_ulp(data), or None; refmix: the oracle's mixed mode
+    (Float32 iterations, Float64 final pass), or None.  Returns a report."""
     e = _errors(model_id, gpu, ref32, ref64, precision)
     # the reference's own Float32 path against the Float64 oracle (its LLR is
     # compared like the GPU's in the same precision mode)
     eref = _errors(model_id, ref32, ref32, ref64, "mixed" if precision == "mixed" else "fp32")
     if precision == "mixed":
-        eref["llr"] = np.zeros_like(eref["llr"])   # Float32 LLR is not the mixed-mode quantity
+        # The mixed-mode LLR is the Float64 LLR at the Float32 iterations'
+        # estimate: the oracle's own mixed mode decides whether it can meet
+        # the tolerance against the Float64-everywhere LLR.
+        if refmix is not None:
+            eref["llr"] = np.abs(np.asarray(refmix["llr"], np.float64) - ref64["llr"]) / np.maximum(
+                1.0, np.abs(ref64["llr"]))
+        else:
+            eref["llr"] = np.zeros_like(eref["llr"])
     well = ~_bad(eref)
     if ref64p is not None:
         # positions of the perturbed Float64 fit against the Float64 oracle

@@ -107,7 +107,8 @@ def test_parity_host_path(name, precision):
     r32, r64, r64p = reference(name, refcpu.FP32), reference(name, refcpu.FP64), reference(name, "fp64p")
     for lanes in lanes_for(w.box):
         print(f"[case] {name} {precision} lanes={lanes} n={w.n}", flush=True)
-        rep = compare(w.psf_pixels.model_id, gpu_fit_host(w, precision, lanes), r32, r64, precision, r64p)
+        rmix = reference(name, refcpu.MIXED) if precision == "mixed" else None
+        rep = compare(w.psf_pixels.model_id, gpu_fit_host(w, precision, lanes), r32, r64, precision, r64p, rmix)
         _report(f"{name} {precision} L={lanes}", rep)
         assert_parity(rep, f"{name} {precision} L={lanes}")
 
