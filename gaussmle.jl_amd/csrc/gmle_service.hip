@@ -4,12 +4,13 @@
 //   * pinned registry: gmle_host_alloc hands out mapped pinned memory and
 //     remembers its device address, so request buffers can be read and written
 //     by kernels directly (no staging copies);
-//   * slots: a device arena (pixels in electrons, per-ROI metadata, grouping
-//     workspace, results) plus a HIP graph
+//   * slots: a device arena (raw counts, pixels in electrons, per-ROI
+//     metadata, grouping workspace, results) plus a HIP graph
 //         ingest -> init -> scatter -> { fit[shape] (-> final[shape]) } -> results
 //     whose kernel nodes are re-parameterised for every batch;
 //   * a dispatcher thread: packs pending requests FIFO into a free slot while
-//     they fit, writes the slot's request / chunk tables (pinned), updates the
+//     they fit, copies their raw counts into the arena (DMA, one copy per
+//     request), writes the slot's request / chunk tables (pinned), updates the
 //     graph nodes and launches it on the slot's stream;
 //   * one waiter thread per slot: waits for the slot's completion event, marks
 //     its requests done, frees the slot.
@@ -84,7 +85,7 @@ struct ReqDesc {              // one request of a batch (device-visible addresse
   const int8_t* boxes;
   const int32_t* xc;
   const int32_t* yc;
-  const uint16_t* adu;
+  const uint16_t* adu;        // the request's raw counts, copied into the arena
   float* params;
   float* crlb;
   float* llr;
@@ -202,7 +203,7 @@ struct Pending {
   const int8_t* boxes = nullptr;
   const int32_t* xc = nullptr;
   const int32_t* yc = nullptr;
-  const uint16_t* adu = nullptr;
+  const uint16_t* adu = nullptr;   // host address (pinned)
   float* params = nullptr;
   float* crlb = nullptr;
   float* llr = nullptr;
@@ -233,6 +234,7 @@ struct Slot {
   std::vector<ShapeNode> shapes;
   // device arena
   char* arena = nullptr;
+  uint16_t* raw = nullptr;   // raw counts of the batch's requests, back to back
   float* data = nullptr;
   int8_t* models = nullptr;
   int8_t* boxes = nullptr;
@@ -289,7 +291,7 @@ struct gmle_service {
   bool stop = false;
   std::thread dispatcher;
   // per-batch statistics (sums)
-  double stat_batches = 0, stat_rois = 0, stat_reqs = 0, stat_host = 0, stat_gpu = 0, stat_seen = 0;
+  double stat_batches = 0, stat_rois = 0, stat_reqs = 0, stat_host = 0, stat_gpu = 0, stat_seen = 0, stat_start = 0;
   int launch_error = GMLE_OK;
 };
 
@@ -297,8 +299,8 @@ namespace gmle {
 namespace {
 
 int64_t roi_bytes(int max_box, bool mixed) {
-  // pixels + models/boxes + corners + offsets + order + params/crlb + llr (+ FP64 outputs)
-  return (int64_t)max_box * max_box * 4 + 2 + 8 + 8 + 4 + 2 * kBatchStride * 4 + 4 +
+  // raw counts + pixels + models/boxes + corners + offsets + order + params/crlb + llr (+ FP64 outputs)
+  return (int64_t)max_box * max_box * 6 + 2 + 8 + 8 + 4 + 2 * kBatchStride * 4 + 4 +
          (mixed ? kBatchStride * 8 + 8 : 0);
 }
 
@@ -375,6 +377,7 @@ int build_slot(gmle_service* s, Slot& sl, size_t slot_bytes) {
   const int64_t C = s->cap_rois;
   if ((e = hipMalloc(&sl.arena, slot_bytes)) != hipSuccess) return hip_fail(e, "hipMalloc service arena");
   char* p = sl.arena;
+  sl.raw = carve<uint16_t>(p, (size_t)s->cap_pix);
   sl.data = carve<float>(p, (size_t)(s->cap_pix + kPixSlack));
   sl.models = carve<int8_t>(p, C);
   sl.boxes = carve<int8_t>(p, C);
@@ -458,7 +461,7 @@ int launch_slot(gmle_service* s, Slot& sl) {
     d.boxes = q.boxes;
     d.xc = q.xc;
     d.yc = q.yc;
-    d.adu = q.adu;
+    d.adu = sl.raw + pix;
     d.params = q.params;
     d.crlb = q.crlb;
     d.llr = q.llr;
@@ -505,6 +508,13 @@ int launch_slot(gmle_service* s, Slot& sl) {
     }
   }
   if ((e = hipEventRecord(sl.start, sl.stream)) != hipSuccess) return hip_fail(e, "hipEventRecord");
+  // Raw counts over PCIe by DMA, then the graph.
+  for (size_t r = 0; r < sl.batch.size(); ++r) {
+    const Pending& q = *sl.batch[r];
+    if ((e = hipMemcpyAsync(sl.raw + (sl.reqs_h[r].pix_base), q.adu, (size_t)q.npix * sizeof(uint16_t),
+                            hipMemcpyHostToDevice, sl.stream)) != hipSuccess)
+      return hip_fail(e, "hipMemcpyAsync (request counts)");
+  }
   if ((e = hipGraphLaunch(sl.exec, sl.stream)) != hipSuccess) return hip_fail(e, "hipGraphLaunch");
   if ((e = hipEventRecord(sl.done, sl.stream)) != hipSuccess) return hip_fail(e, "hipEventRecord");
   return GMLE_OK;
@@ -576,7 +586,9 @@ void waiter_main(gmle_service* s, Slot* sl) {
     // Poll rather than block: a blocking wait's wake-up costs more than a
     // small batch takes on the GPU.
     hipError_t e;
-    while ((e = hipEventQuery(sl->done)) == hipErrorNotReady) std::this_thread::yield();
+    while ((e = hipEventQuery(sl->start)) == hipErrorNotReady) std::this_thread::yield();
+    const double t_started = now_s();
+    while (e == hipSuccess && (e = hipEventQuery(sl->done)) == hipErrorNotReady) std::this_thread::yield();
     const double t_seen = now_s();
     float gpu_ms = 0.f;
     if (e == hipSuccess) (void)hipEventElapsedTime(&gpu_ms, sl->start, sl->done);
@@ -589,6 +601,7 @@ void waiter_main(gmle_service* s, Slot* sl) {
     s->stat_host += sl->t_launched - sl->t_formed;
     s->stat_gpu += gpu_ms * 1e-3;
     s->stat_seen += t_seen - sl->t_launched;
+    s->stat_start += t_started - sl->t_launched;
     lock.unlock();
     finish(s, b, e == hipSuccess ? GMLE_OK : GMLE_ERR_HIP);
     lock.lock();
@@ -635,7 +648,7 @@ int make_pending(gmle_service* s, const gmle_request* r, PendingP& out) {
   if (npix > s->cap_pix) return fail(GMLE_ERR_ARG, "request pixels exceed a slot's capacity");
   q->npix = npix;
   if (!pinned_dev(r->adu, 2 * npix, &d)) return fail(GMLE_ERR_ARG, "request pixels are not in gmle_host_alloc memory");
-  q->adu = (const uint16_t*)d;
+  q->adu = r->adu;
   if (!pinned_dev(r->params, 4 * kBatchStride * n, &d)) return fail(GMLE_ERR_ARG, "params not in gmle_host_alloc memory");
   q->params = (float*)d;
   if (!pinned_dev(r->crlb, 4 * kBatchStride * n, &d)) return fail(GMLE_ERR_ARG, "crlb not in gmle_host_alloc memory");
@@ -891,9 +904,9 @@ int gmle_service_stats(gmle_service* s, double* out, int n) {
   if (!s || !out) return fail(GMLE_ERR_ARG, "NULL argument");
   std::lock_guard<std::mutex> lock(s->mu);
   const double b = s->stat_batches > 0 ? s->stat_batches : 1.0;
-  const double v[6] = {s->stat_batches, s->stat_rois / b, s->stat_reqs / b, s->stat_host / b * 1e6,
-                       s->stat_gpu / b * 1e6, s->stat_seen / b * 1e6};
-  for (int k = 0; k < n && k < 6; ++k) out[k] = v[k];
+  const double v[7] = {s->stat_batches, s->stat_rois / b, s->stat_reqs / b, s->stat_host / b * 1e6,
+                       s->stat_gpu / b * 1e6, s->stat_seen / b * 1e6, s->stat_start / b * 1e6};
+  for (int k = 0; k < n && k < 7; ++k) out[k] = v[k];
   return GMLE_OK;
 }
 

@@ -244,7 +244,8 @@ int gmle_service_info(gmle_service* s, int64_t* slot_rois, int64_t* slot_pixels,
 
 /* Per-batch averages since creation, up to n of: batches, ROIs per batch,
  * requests per batch, host time to form and launch a batch (us), GPU time of
- * a batch (us), time from launch to the completion being seen (us). */
+ * a batch (us), time from launch to the completion being seen (us), time from
+ * launch to the batch's start on the GPU being seen (us). */
 int gmle_service_stats(gmle_service* s, double* out, int n);
 
 /* Waits for in-flight work, then frees everything. */
