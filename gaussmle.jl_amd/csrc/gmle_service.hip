@@ -29,6 +29,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -293,6 +294,7 @@ struct gmle_service {
   // per-batch statistics (sums)
   double stat_batches = 0, stat_rois = 0, stat_reqs = 0, stat_host = 0, stat_gpu = 0, stat_seen = 0, stat_start = 0;
   int launch_error = GMLE_OK;
+  std::string error_msg;        // first failure of a batch (launch or completion)
 };
 
 namespace gmle {
@@ -564,6 +566,11 @@ void dispatcher_main(gmle_service* s) {
     lock.lock();
     if (rc != GMLE_OK) {
       s->launch_error = rc;
+      if (s->error_msg.empty()) {
+        char buf[512];
+        gmle_last_error(buf, sizeof(buf));
+        s->error_msg = buf;
+      }
       lock.unlock();
       finish(s, sl.batch, rc);
       lock.lock();
@@ -588,12 +595,16 @@ void waiter_main(gmle_service* s, Slot* sl) {
     hipError_t e;
     while ((e = hipEventQuery(sl->start)) == hipErrorNotReady) std::this_thread::yield();
     const double t_started = now_s();
-    while (e == hipSuccess && (e = hipEventQuery(sl->done)) == hipErrorNotReady) std::this_thread::yield();
+    if (e == hipSuccess) {
+      while ((e = hipEventQuery(sl->done)) == hipErrorNotReady) std::this_thread::yield();
+    }
     const double t_seen = now_s();
     float gpu_ms = 0.f;
     if (e == hipSuccess) (void)hipEventElapsedTime(&gpu_ms, sl->start, sl->done);
     std::vector<PendingP> b;
     lock.lock();
+    if (e != hipSuccess && s->error_msg.empty())
+      s->error_msg = std::string("batch completion: ") + hipGetErrorString(e);
     b.swap(sl->batch);
     s->stat_batches += 1;
     s->stat_rois += (double)sl->batch_rois;
@@ -847,7 +858,8 @@ int gmle_service_wait(gmle_service* s, int64_t ticket, double timeout_s) {
   const auto ok = s->cv_done.wait_for(lock, std::chrono::duration<double>(timeout_s), [&] { return q->done; });
   if (!ok) return 1;
   s->tickets.erase(ticket);
-  return q->status;
+  if (q->status != GMLE_OK) return fail(q->status, "a batch failed: %s", s->error_msg.c_str());
+  return GMLE_OK;
 }
 
 int gmle_service_replay(gmle_service* s, int64_t n, const gmle_request* reqs, const double* arrival_s,
@@ -887,7 +899,9 @@ int gmle_service_replay(gmle_service* s, int64_t n, const gmle_request* reqs, co
   }
   if (elapsed_s) *elapsed_s = last - t0;
   (void)t1;
-  return status == GMLE_OK ? GMLE_OK : fail(status, "a batch failed to launch or complete");
+  if (status == GMLE_OK) return GMLE_OK;
+  std::lock_guard<std::mutex> lock(s->mu);
+  return fail(status, "a batch failed: %s", s->error_msg.c_str());
 }
 
 int gmle_service_info(gmle_service* s, int64_t* slot_rois, int64_t* slot_pixels, int* slots, int64_t* batches) {
