@@ -98,19 +98,34 @@ bool lanes_ok(int box, int lanes) {
   return box >= kMinBox && box <= kMaxBox && ((lanes == 1 && box <= 12) || lanes == 2 || lanes == 4);
 }
 
-// Lanes per ROI when the caller passes 0.  One lane per ROI keeps the
-// per-ROI work free of cross-lane traffic and is the fastest once the launch
-// has enough waves to fill the chip; smaller batches trade that for more
-// waves (more lanes per ROI), and large boxes need more lanes to keep the ROI
-// resident in VGPRs.
+// Lanes per ROI for a large batch: the fewest lanes whose kernel still runs
+// two or more waves per SIMD without spilling.  A lane holds its ceil(box/L)
+// pixel columns (and, for sCMOS, their variances) in VGPRs for the whole fit,
+// so the register footprint grows with box / L, faster with sigma / z terms
+// and doubles its pixel share with sCMOS.  Thresholds from the gfx950 build's
+// register counts (kernel-resource-usage); the batch autotuner measures.
+int large_batch_lanes(int model, int camera, int box) {
+  int one, two;   // largest box served with 1 lane, with 2 lanes
+  const bool sc = camera == SCMOS;
+  switch (model) {
+    case XYNB: one = sc ? 6 : 12; two = sc ? 11 : 14; break;
+    case XYNBS: one = sc ? 6 : 7; two = sc ? 9 : 11; break;
+    case XYNBSXSY: one = sc ? 6 : 8; two = sc ? 10 : 12; break;
+    default: one = sc ? 5 : 9; two = sc ? 8 : 11; break;   // ASTIG
+  }
+  return box <= one ? 1 : box <= two ? 2 : 4;
+}
+
+// Lanes per ROI when the caller passes 0: the large-batch choice, raised while
+// the launch has too few waves to fill the chip (more lanes per ROI give more,
+// shorter waves).
 int default_lanes(int model, int camera, int box, int precision, int64_t n_rois) {
-  (void)model;
-  (void)camera;
   (void)precision;
   const int64_t fill = 256 * 4 * 2 * 64;   // CUs x SIMDs x waves x lanes
-  int lanes = box <= 12 ? 1 : 2;
-  if (n_rois * lanes < fill && lanes_ok(box, 2)) lanes = 2;
-  if (n_rois * lanes < fill && lanes_ok(box, 4)) lanes = 4;
+  int lanes = large_batch_lanes(model, camera, box);
+  if (!lanes_ok(box, lanes)) lanes = 2;
+  if (lanes < 2 && n_rois * lanes < fill && lanes_ok(box, 2)) lanes = 2;
+  if (lanes < 4 && n_rois * lanes < fill && lanes_ok(box, 4)) lanes = 4;
   return lanes;
 }
 
