@@ -129,11 +129,13 @@ def run_mixed(n: int, steps: int, warmup: int, precision: str, tune_reps: int = 
 
     wall_b, dev_b = timed(lambda: batched(tuning.streams))
     wall_1, dev_1 = timed(lambda: batched(1))
+    wall_0, dev_0 = timed(lambda: batched(0))
     wall_p, dev_p = timed(plain)
     res = {
         "workload": MIXED_LABEL, "rois": n, "precision": precision,
         "batched_fits_per_s": round(steps * n / wall_b, 1),
         "batched_one_stream_fits_per_s": round(steps * n / wall_1, 1),
+        "one_launch_fits_per_s": round(steps * n / wall_0, 1),
         "per_shape_plain_fits_per_s": round(steps * n / wall_p, 1),
         "batched_over_plain": round(wall_p / wall_b, 4),
         "batched_ms": round(dev_b * 1e3, 4), "plain_ms": round(dev_p * 1e3, 4),

@@ -21,6 +21,7 @@
 #include "gmle_args.hpp"
 #include "gmle_batch.hpp"
 #include "gmle_host.hpp"
+#include "gmle_multi.hpp"
 
 namespace gmle {
 
@@ -296,8 +297,10 @@ int check_batch(const BatchSpec& b) {
   if (b.precision != PREC_FP32 && b.precision != PREC_MIXED)
     return fail(GMLE_ERR_ARG, "unknown precision %d", b.precision);
   if (b.iterations < 0) return fail(GMLE_ERR_ARG, "iterations must be >= 0, got %d", b.iterations);
-  if (b.streams < 1 || b.streams > kMaxStreams)
-    return fail(GMLE_ERR_ARG, "streams must be 1..%d, got %d", kMaxStreams, b.streams);
+  if (b.streams < 0 || b.streams > kMaxStreams)
+    return fail(GMLE_ERR_ARG, "streams must be 0..%d, got %d", kMaxStreams, b.streams);
+  if (b.streams == 0 && b.precision != PREC_FP32)
+    return fail(GMLE_ERR_UNSUPPORTED, "the one-launch mode (streams = 0) runs Float32 only");
   if (!b.psf || !b.lower || !b.upper || !b.max_step)
     return fail(GMLE_ERR_ARG, "psf and constraints arrays must not be NULL");
   int64_t total = 0;
@@ -348,6 +351,48 @@ int enqueue_batch(const BatchSpec& b, hipStream_t stream) {
   hipError_t e = enqueue_batch_plan(p, stream);
   if (e != hipSuccess) return hip_fail(e, "batch planning launch");
 
+  if (b.streams == 0) {
+    // One launch for every eligible shape (4 lanes per ROI), heaviest tiles
+    // first; the others follow on the same stream with their own kernels.
+    std::vector<int> multi, rest;
+    for (int s : shapes)
+      (multi_eligible(s / kNumBoxes, b.camera, kMinBox + s % kNumBoxes) ? multi : rest).push_back(s);
+    std::sort(multi.begin(), multi.end(), [&](int u, int v) {
+      return shape_cost(u / kNumBoxes, kMinBox + u % kNumBoxes) > shape_cost(v / kNumBoxes, kMinBox + v % kNumBoxes);
+    });
+    MultiArgs m;
+    std::memset(&m, 0, sizeof(m));
+    for (int model = 0; model < 4; ++model) {
+      FitArgs& a = m.fa[model];
+      fill_args(a, model, b.psf + 8 * model, b.lower + 6 * model, b.upper + 6 * model, b.max_step + 6 * model, 0,
+                b.ny, b.nx, b.iterations);
+      a.data = b.data;
+      a.var_map = b.camera == SCMOS ? b.var_map : nullptr;
+      a.x_corners = b.camera == SCMOS ? b.xc : nullptr;
+      a.y_corners = b.camera == SCMOS ? b.yc : nullptr;
+      a.params = b.params;
+      a.crlb = b.crlb;
+      a.llr = b.llr;
+      a.order = p.order;
+      a.offsets = b.offsets;
+      a.out_stride = kBatchStride;
+    }
+    int32_t tiles = 0;
+    for (int s : multi) {
+      const int q = m.n_entries++;
+      m.shape[q] = s;
+      m.first[q] = p.start[s];
+      m.count[q] = p.count[s];
+      m.tile_start[q] = tiles;
+      tiles += (p.count[s] + kMultiTile - 1) / kMultiTile;
+    }
+    m.tile_start[m.n_entries] = tiles;
+    if (m.n_entries > 0) {
+      e = b.camera == SCMOS ? launch_multi_scmos(m, stream) : launch_multi_ideal(m, stream);
+      if (e != hipSuccess) return hip_fail(e, "kernel launch");
+    }
+    shapes = rest;
+  }
   std::sort(shapes.begin(), shapes.end(), [&](int u, int v) {
     const double cu = b.counts[u] * shape_cost(u / kNumBoxes, kMinBox + u % kNumBoxes);
     const double cv = b.counts[v] * shape_cost(v / kNumBoxes, kMinBox + v % kNumBoxes);

@@ -38,6 +38,7 @@
 #include "gmle_args.hpp"
 #include "gmle_batch.hpp"
 #include "gmle_host.hpp"
+#include "gmle_multi.hpp"
 
 namespace gmle {
 namespace {
@@ -232,7 +233,9 @@ struct Slot {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipGraphNode_t n_ingest = nullptr, n_init = nullptr, n_scatter = nullptr, n_results = nullptr;
-  std::vector<ShapeNode> shapes;
+  hipGraphNode_t n_multi = nullptr;   // every shape served by the multi-shape kernel (Float32)
+  std::vector<int> multi_shapes;
+  std::vector<ShapeNode> shapes;      // shapes with a kernel of their own
   // device arena
   char* arena = nullptr;
   uint16_t* raw = nullptr;   // raw counts of the batch's requests, back to back
@@ -424,8 +427,23 @@ int build_slot(gmle_service* s, Slot& sl, size_t slot_bytes) {
       (e = add_kernel_node(&sl.n_scatter, sl.graph, {sl.n_init}, batch_scatter_kernel(), kBatchBlock, &bp)) != hipSuccess)
     return hip_fail(e, "hipGraphAddKernelNode");
   std::vector<hipGraphNode_t> tails;
+  // Float32: one multi-shape node (4 lanes per ROI) for every shape it can
+  // hold; HIP runs a graph's kernel nodes one after another, so one node for
+  // all shapes rather than one per shape.
+  const bool use_multi = s->precision == PREC_FP32;
+  if (use_multi) {
+    MultiArgs m;
+    std::memset(&m, 0, sizeof(m));
+    if ((e = add_kernel_node(&sl.n_multi, sl.graph, {sl.n_scatter}, multi_kernel_scmos(), kWave, &m)) != hipSuccess)
+      return hip_fail(e, "hipGraphAddKernelNode");
+    tails.push_back(sl.n_multi);
+  }
   for (int sh = 0; sh < kNumShapes; ++sh) {
     if (s->lanes[sh] <= 0) continue;
+    if (use_multi && multi_eligible(sh / kNumBoxes, SCMOS, kMinBox + sh % kNumBoxes)) {
+      sl.multi_shapes.push_back(sh);
+      continue;
+    }
     ShapeNode n;
     n.shape = sh;
     n.model = sh / kNumBoxes;
@@ -444,6 +462,10 @@ int build_slot(gmle_service* s, Slot& sl, size_t slot_bytes) {
     }
     sl.shapes.push_back(n);
   }
+  std::sort(sl.multi_shapes.begin(), sl.multi_shapes.end(), [](int u, int v) {
+    const int bu = kMinBox + u % kNumBoxes, bv = kMinBox + v % kNumBoxes;
+    return bu * bu * (u / kNumBoxes + 4) > bv * bv * (v / kNumBoxes + 4);
+  });
   if ((e = add_kernel_node(&sl.n_results, sl.graph, tails, (const void*)&service_results, kIngestBlock, &ra)) != hipSuccess)
     return hip_fail(e, "hipGraphAddKernelNode");
   if ((e = hipGraphInstantiate(&sl.exec, sl.graph, nullptr, nullptr, 0)) != hipSuccess)
@@ -497,6 +519,29 @@ int launch_slot(gmle_service* s, Slot& sl) {
       (e = set_kernel_node(sl.exec, sl.n_scatter, batch_scatter_kernel(), gb, kBatchBlock, &bp)) != hipSuccess ||
       (e = set_kernel_node(sl.exec, sl.n_results, (const void*)&service_results, (unsigned)nch, kIngestBlock, &ra)) != hipSuccess)
     return hip_fail(e, "hipGraphExecKernelNodeSetParams");
+  if (sl.n_multi) {
+    MultiArgs m;
+    std::memset(&m, 0, sizeof(m));
+    for (int model = 0; model < 4; ++model) {
+      ShapeNode n;
+      n.model = model;
+      m.fa[model] = shape_args(s, sl, n, 0, 0);
+    }
+    int32_t tiles = 0;
+    for (int sh : sl.multi_shapes) {   // heaviest shapes' tiles first
+      if (counts[sh] == 0) continue;
+      const int q = m.n_entries++;
+      m.shape[q] = sh;
+      m.first[q] = bp.start[sh];
+      m.count[q] = bp.count[sh];
+      m.tile_start[q] = tiles;
+      tiles += (bp.count[sh] + kMultiTile - 1) / kMultiTile;
+    }
+    m.tile_start[m.n_entries] = tiles;
+    if ((e = set_kernel_node(sl.exec, sl.n_multi, multi_kernel_scmos(), (unsigned)std::max(1, tiles), kWave, &m)) !=
+        hipSuccess)
+      return hip_fail(e, "hipGraphExecKernelNodeSetParams");
+  }
   for (const ShapeNode& n : sl.shapes) {
     const int64_t cnt = counts[n.shape];
     FitArgs fa = shape_args(s, sl, n, bp.start[n.shape], cnt);

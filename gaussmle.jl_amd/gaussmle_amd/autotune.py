@@ -9,7 +9,8 @@ batch also on what runs beside it.  The tuner:
      batch (plain launches, device-resident);
   2. refines in the batch itself: coordinate descent over the shapes, timing
      the whole batched call, keeping a change only when it is faster;
-  3. picks the number of streams the per-shape launches are spread over.
+  3. picks the number of streams the per-shape launches are spread over, or
+     the one-launch mode (every shape in one multi-shape kernel, 4 lanes).
 All timings are HIP-event times on the device, median of `reps` runs.
 """
 from __future__ import annotations
@@ -76,6 +77,7 @@ class Tuning:
     streams: int
     alone_ms: dict = field(default_factory=dict)     # (model, box) -> {L: ms}
     batch_ms: float = 0.0
+    one_launch_ms: float = 0.0
     history: list = field(default_factory=list)
 
     def table(self) -> np.ndarray:
@@ -86,7 +88,7 @@ class Tuning:
 
     def to_json(self) -> dict:
         return {"lanes": {f"{m}:{b}": L for (m, b), L in self.lanes.items()}, "streams": self.streams,
-                "batch_ms": self.batch_ms,
+                "batch_ms": self.batch_ms, "one_launch_ms": self.one_launch_ms,
                 "alone_ms": {f"{m}:{b}": {str(L): t for L, t in v.items()} for (m, b), v in self.alone_ms.items()}}
 
     def save(self, path):
@@ -139,5 +141,12 @@ def tune(mb: gb.MixedBatch, d: dict, precision: str = "fp32", reps: int = 3, sta
         tuning.history.append((f"streams{s}", t))
         if t < cur * 0.99:
             tuning.streams, cur = s, t
+    if precision == "fp32":
+        # one launch for every shape (4 lanes per ROI; the lanes table is unused)
+        t = batch_ms(tuning.lanes, 0)
+        tuning.history.append(("one_launch", t))
+        tuning.one_launch_ms = t
+        if t < cur * 0.99:
+            tuning.streams, cur = 0, t
     tuning.batch_ms = cur
     return tuning

@@ -136,6 +136,10 @@ int gmle_fit_device(int model, const float* psf, int box, int64_t n_rois, const 
  *                  lanes per ROI of each shape (0 = automatic), or NULL.
  *   streams        1-4: the per-shape launches are spread over this many HIP
  *                  streams so that shapes overlap on the GPU (1 = in order).
+ *                  0: one launch for every shape, with 4 lanes per ROI (the
+ *                  lanes table is then ignored); shapes whose 4-lane kernel
+ *                  needs more than 256 VGPRs (astigmatic sCMOS ROIs of 14 px
+ *                  and up) keep their own launch.  GMLE_FP32 only.
  * Every ROI's result is bit-identical to a gmle_fit call on that ROI alone with
  * the same model, box, lanes and precision.
  * ------------------------------------------------------------------------- */

@@ -181,3 +181,18 @@ def test_batch_unknown_shapes_and_short_counts_give_nan():
     keep[bad] = False
     np.testing.assert_array_equal(r["params"][:, keep], ref.params[:, keep])
     np.testing.assert_array_equal(r["llr"][keep], ref.llr[keep])
+
+
+@pytest.mark.parametrize("name", ["scmos", "ideal"])
+def test_batch_one_launch_mode_matches_four_lane_plain(name):
+    """streams = 0: every shape in one multi-shape launch, 4 lanes per ROI."""
+    mb = mixed(name)
+    d = gb.upload_batch(mb)
+    out = gb.fit_batch_device(mb, d, "fp32", None, streams=0)
+    res = device_result(out, "fp32")
+    assert_same_as_plain(mb, res, "fp32", {k: 4 for k in resolved_lanes(mb, np.zeros(gb.NUM_SHAPES, np.int32),
+                                                                        "fp32")})
+    h = gb.fit_batch(mb, "fp32", None, streams=0)
+    np.testing.assert_array_equal(h.params, res["params"])
+    np.testing.assert_array_equal(h.llr, res["llr"])
+

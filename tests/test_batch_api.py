@@ -58,7 +58,7 @@ def test_fit_batch_rejects_bad_input(hiplib):
     bad.x_corners[3] = 32
     assert call(bad) == _lib.GMLE_ERR_BOUNDS and "ROI 4" in _lib.last_error()
     bad.x_corners[3] = mb.x_corners[3]
-    assert call(bad, streams=0) == _lib.GMLE_ERR_ARG
+    assert call(bad, streams=-1) == _lib.GMLE_ERR_ARG
     assert call(bad, streams=5) == _lib.GMLE_ERR_ARG
     assert call(bad, iterations=-1) == _lib.GMLE_ERR_ARG
     lanes = np.zeros(_lib.NUM_SHAPES, np.int32)
