@@ -11,6 +11,7 @@ from __future__ import annotations
 import argparse
 import concurrent.futures as cf
 import os
+import re
 import shutil
 import subprocess
 import sys
@@ -55,9 +56,19 @@ def _compile(src: Path, force: bool, hdr_mtime: float) -> Path:
     if not force and obj.exists() and obj.stat().st_mtime >= max(src.stat().st_mtime, hdr_mtime):
         return obj
     cmd = [hipcc(), *HIPFLAGS, "-I", str(INCLUDE), "-c", str(src), "-o", str(obj)]
+    multi = src.name.startswith("inst_multi_")
+    if multi:
+        cmd.append("-Rpass-analysis=kernel-resource-usage")
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src.name}:\n{r.stderr[-4000:]}")
+    if multi:
+        # The multi-shape kernel is sized to run two waves per SIMD: its tile
+        # functions' register limits (max_box_within_256) assume this compiler.
+        occ = re.findall(r"Occupancy \[waves/SIMD\]: (\d+)", r.stderr)
+        if occ and min(int(o) for o in occ) < 2:
+            print(f"[build] warning: {src.name}: multi-shape kernel at {min(int(o) for o in occ)} wave(s) per SIMD; "
+                  "lower the limits in max_box_within_256", flush=True)
     return obj
 
 

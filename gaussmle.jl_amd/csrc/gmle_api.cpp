@@ -100,21 +100,9 @@ bool lanes_ok(int box, int lanes) {
 }
 
 // Lanes per ROI for a large batch: the fewest lanes whose kernel still runs
-// two or more waves per SIMD without spilling.  A lane holds its ceil(box/L)
-// pixel columns (and, for sCMOS, their variances) in VGPRs for the whole fit,
-// so the register footprint grows with box / L, faster with sigma / z terms
-// and doubles its pixel share with sCMOS.  Thresholds from the gfx950 build's
-// register counts (kernel-resource-usage); the batch autotuner measures.
+// two or more waves per SIMD without spilling (the batch autotuner measures).
 int large_batch_lanes(int model, int camera, int box) {
-  int one, two;   // largest box served with 1 lane, with 2 lanes
-  const bool sc = camera == SCMOS;
-  switch (model) {
-    case XYNB: one = sc ? 6 : 12; two = sc ? 11 : 14; break;
-    case XYNBS: one = sc ? 6 : 7; two = sc ? 9 : 11; break;
-    case XYNBSXSY: one = sc ? 6 : 8; two = sc ? 10 : 12; break;
-    default: one = sc ? 5 : 9; two = sc ? 8 : 11; break;   // ASTIG
-  }
-  return box <= one ? 1 : box <= two ? 2 : 4;
+  return box <= max_box_within_256(model, camera, 1) ? 1 : box <= max_box_within_256(model, camera, 2) ? 2 : 4;
 }
 
 // Lanes per ROI when the caller passes 0: the large-batch choice, raised while
@@ -352,8 +340,9 @@ int enqueue_batch(const BatchSpec& b, hipStream_t stream) {
   if (e != hipSuccess) return hip_fail(e, "batch planning launch");
 
   if (b.streams == 0) {
-    // One launch for every eligible shape (4 lanes per ROI), heaviest tiles
-    // first; the others follow on the same stream with their own kernels.
+    // One launch for every shape the multi-shape kernel holds, heaviest tiles
+    // first, with the requested lanes where it has them (else 4); the others
+    // follow on the same stream with their own kernels.
     std::vector<int> multi, rest;
     for (int s : shapes)
       (multi_eligible(s / kNumBoxes, b.camera, kMinBox + s % kNumBoxes) ? multi : rest).push_back(s);
@@ -379,12 +368,16 @@ int enqueue_batch(const BatchSpec& b, hipStream_t stream) {
     }
     int32_t tiles = 0;
     for (int s : multi) {
+      const int model = s / kNumBoxes, box = kMinBox + s % kNumBoxes;
+      const int want = (b.lanes && b.lanes[s] > 0) ? b.lanes[s] : large_batch_lanes(model, b.camera, box);
+      const int L = multi_lanes(model, b.camera, box, want);
       const int q = m.n_entries++;
       m.shape[q] = s;
+      m.lanes[q] = L;
       m.first[q] = p.start[s];
       m.count[q] = p.count[s];
       m.tile_start[q] = tiles;
-      tiles += (p.count[s] + kMultiTile - 1) / kMultiTile;
+      tiles += (p.count[s] + kWave / L - 1) / (kWave / L);
     }
     m.tile_start[m.n_entries] = tiles;
     if (m.n_entries > 0) {
@@ -634,6 +627,12 @@ int gmle_fit(int model, const float* psf, int box, int64_t n_rois, const float* 
 }
 
 int gmle_shape_index(int model, int box) { return shape_index(model, box); }
+
+int gmle_one_launch_lanes(int model, int camera, int box, int lanes) {
+  if (shape_index(model, box) < 0 || (camera != IDEAL && camera != SCMOS)) return 0;
+  if (!multi_eligible(model, camera, box)) return 0;
+  return multi_lanes(model, camera, box, lanes > 0 ? lanes : large_batch_lanes(model, camera, box));
+}
 
 size_t gmle_batch_workspace_bytes(int64_t n_rois) { return batch_workspace(n_rois < 0 ? 0 : n_rois); }
 

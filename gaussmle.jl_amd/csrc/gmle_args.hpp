@@ -19,6 +19,24 @@ __host__ __device__ constexpr int n_params(int model) {
   return model == XYNB ? 4 : model == XYNBS ? 5 : model == XYNBSXSY ? 6 : 5;
 }
 
+// Largest box size whose kernel with `lanes` lanes per ROI (1 or 2) still
+// fits 256 VGPRs without spilling, i.e. runs two or more waves per SIMD, with
+// a few registers to spare for the multi-shape kernel's call.  A
+// lane keeps its ceil(box / lanes) pixel columns (and, for sCMOS, their
+// variances) in VGPRs for the whole fit.  From the gfx950 build's register
+// counts (kernel-resource-usage); 4 lanes fit every box except astigmatic
+// sCMOS ROIs of 14 px and up.
+__host__ __device__ constexpr int max_box_within_256(int model, int camera, int lanes) {
+  const bool sc = camera == SCMOS;
+  if (lanes == 1)
+    return model == XYNB ? (sc ? 6 : 11) : model == XYNBS ? (sc ? 5 : 6) : model == XYNBSXSY ? (sc ? 5 : 7)
+                                                                                             : (sc ? 5 : 8);
+  if (lanes == 2)
+    return model == XYNB ? (sc ? 11 : 14) : model == XYNBS ? (sc ? 9 : 11) : model == XYNBSXSY ? (sc ? 9 : 11)
+                                                                                               : (sc ? 8 : 10);
+  return (model == ASTIG && sc) ? 13 : kMaxBox;
+}
+
 // Passed by value to every fitting kernel.  Arrays follow the reference's
 // Julia layout: data (box, box, n) column-major with ROI k at offset box*box*k;
 // params / crlb (P, n) with ROI k's P values contiguous; var_map (ny, nx)

@@ -1,13 +1,14 @@
 // gmle_multi.hpp -- every shape of a mixed batch in one launch.
 //
-// multi_fit<CAM> runs tiles of 16 ROIs (4 lanes per ROI) of any (model, box)
-// shape: block t finds its shape in the launch's tile table (wave-uniform) and
-// calls that shape's tile function.  The tile functions are separate,
-// non-inlined functions, so each is register-allocated on its own: the kernel
-// needs the largest of them (at most 256 VGPRs with 4 lanes per ROI, two
-// waves per SIMD) instead of what one body holding every shape would need.
+// multi_fit<CAM> runs tiles of 64/L ROIs (L lanes per ROI) of any
+// (model, box) shape: block t finds its shape in the launch's tile table
+// (wave-uniform) and calls that shape's tile function.  The tile functions
+// are separate, non-inlined functions, so each is register-allocated on its
+// own: the kernel needs the largest of them (at most 256 VGPRs, two waves per
+// SIMD) instead of what one body holding every shape would need.  Each shape
+// offers its 4-lane tile and, where they fit 256 VGPRs, 2- and 1-lane tiles.
 // The per-ROI code is fit_one, as in the per-shape kernels, so results are
-// bit-identical to a per-shape launch with 4 lanes per ROI.
+// bit-identical to a per-shape launch with the same lanes per ROI.
 #pragma once
 
 #include "gmle_batch.hpp"
@@ -15,33 +16,39 @@
 
 namespace gmle {
 
-constexpr int kMultiLanes = 4;
-constexpr int kMultiTile = kWave / kMultiLanes;   // ROIs per tile
-
-// Shapes whose 4-lane kernel stays within 256 VGPRs (two waves per SIMD);
-// the others (astigmatic sCMOS ROIs of 14 px and up) keep their own launch.
+// Shapes and lanes the multi-shape kernel holds: those whose kernel stays
+// within 256 VGPRs (two waves per SIMD).  Shapes without a 4-lane tile here
+// (astigmatic sCMOS ROIs of 14 px and up) keep their own launch.
+__host__ __device__ constexpr bool multi_has(int model, int camera, int box, int lanes) {
+  return (lanes == 1 || lanes == 2 || lanes == 4) && box >= kMinBox && box <= max_box_within_256(model, camera, lanes);
+}
 __host__ __device__ constexpr bool multi_eligible(int model, int camera, int box) {
-  return !(model == ASTIG && camera == SCMOS && box >= 14);
+  return multi_has(model, camera, box, 4);
+}
+// Lanes used for a shape in the multi-shape kernel when `want` is requested.
+__host__ __device__ constexpr int multi_lanes(int model, int camera, int box, int want) {
+  return multi_has(model, camera, box, want) ? want : 4;
 }
 
 struct MultiArgs {
   FitArgs fa[4];                  // per model: psf / constraints; shared data, order and outputs
   int32_t n_entries;
   int32_t shape[kNumShapes];      // entry q: shape index
+  int32_t lanes[kNumShapes];      //          lanes per ROI (multi_has)
   int32_t first[kNumShapes];      //          its first slot in order[]
   int32_t count[kNumShapes];      //          its slots
   int32_t tile_start[kNumShapes + 1];
 };
 
-template <int MODEL, int CAM, int N>
+template <int MODEL, int CAM, int N, int L>
 __device__ __attribute__((noinline)) void multi_tile(const FitArgs& a, int32_t first, int32_t count, int32_t t) {
-  constexpr int L = kMultiLanes;
+  constexpr int RPT = kWave / L;   // ROIs per tile
   const int lane = threadIdx.x;
-  const int r = lane & (L - 1);
+  const int r = (L == 1) ? 0 : (lane & (L - 1));
   // An entry whose first slot is empty holds no ROI of this shape.
   const int32_t k0 = __builtin_amdgcn_readfirstlane(a.order[first]);
   if (k0 < 0) return;
-  const int32_t j = t * kMultiTile + lane / L;
+  const int32_t j = t * RPT + lane / L;
   const bool in = j < count;
   int32_t k = a.order[first + (in ? j : 0)];
   const bool active = in && k >= 0;
@@ -60,7 +67,16 @@ __device__ __attribute__((noinline)) void multi_tile(const FitArgs& a, int32_t f
 
 template <int CAM, int MODEL, int N>
 __device__ __forceinline__ void multi_case(const MultiArgs& m, int q, int t) {
-  if constexpr (multi_eligible(MODEL, CAM, N)) multi_tile<MODEL, CAM, N>(m.fa[MODEL], m.first[q], m.count[q], t);
+  const int L = m.lanes[q];
+  if constexpr (multi_has(MODEL, CAM, N, 1)) {
+    if (L == 1) { multi_tile<MODEL, CAM, N, 1>(m.fa[MODEL], m.first[q], m.count[q], t); return; }
+  }
+  if constexpr (multi_has(MODEL, CAM, N, 2)) {
+    if (L == 2) { multi_tile<MODEL, CAM, N, 2>(m.fa[MODEL], m.first[q], m.count[q], t); return; }
+  }
+  if constexpr (multi_has(MODEL, CAM, N, 4)) {
+    if (L == 4) multi_tile<MODEL, CAM, N, 4>(m.fa[MODEL], m.first[q], m.count[q], t);
+  }
 }
 
 template <int CAM, int MODEL>

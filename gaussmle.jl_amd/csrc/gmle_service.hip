@@ -46,6 +46,12 @@ namespace {
 constexpr int kChunk = 64;          // ROIs per ingest / result block
 constexpr int kIngestBlock = 256;
 constexpr int kMaxReqPerBatch = 4096;
+// A batch stops growing at this many ROIs (a larger request still forms a
+// batch of its own): past it the GPU is full anyway, and splitting lets one
+// slot's PCIe copies overlap another slot's fits.
+constexpr int64_t kTargetBatchRois = 1 << 16;
+// Completion polling: spin this long, then sleep between polls.
+constexpr double kSpinSeconds = 2e-3;
 // Pinned memory the GPU reads and writes directly: mapped, visible to every
 // device, coherent (not cached on the GPU, so tables the host rewrites for
 // every batch are never read stale).
@@ -296,6 +302,7 @@ struct gmle_service {
   std::thread dispatcher;
   // per-batch statistics (sums)
   double stat_batches = 0, stat_rois = 0, stat_reqs = 0, stat_host = 0, stat_gpu = 0, stat_seen = 0, stat_start = 0;
+  double stat_gpu_max = 0, stat_seen_max = 0;
   int launch_error = GMLE_OK;
   std::string error_msg;        // first failure of a batch (launch or completion)
 };
@@ -427,9 +434,9 @@ int build_slot(gmle_service* s, Slot& sl, size_t slot_bytes) {
       (e = add_kernel_node(&sl.n_scatter, sl.graph, {sl.n_init}, batch_scatter_kernel(), kBatchBlock, &bp)) != hipSuccess)
     return hip_fail(e, "hipGraphAddKernelNode");
   std::vector<hipGraphNode_t> tails;
-  // Float32: one multi-shape node (4 lanes per ROI) for every shape it can
-  // hold; HIP runs a graph's kernel nodes one after another, so one node for
-  // all shapes rather than one per shape.
+  // Float32: one multi-shape node for every shape it can hold (with the
+  // table's lanes where it has them, else 4); HIP runs a graph's kernel nodes
+  // one after another, so one node for all shapes rather than one per shape.
   const bool use_multi = s->precision == PREC_FP32;
   if (use_multi) {
     MultiArgs m;
@@ -530,12 +537,14 @@ int launch_slot(gmle_service* s, Slot& sl) {
     int32_t tiles = 0;
     for (int sh : sl.multi_shapes) {   // heaviest shapes' tiles first
       if (counts[sh] == 0) continue;
+      const int L = multi_lanes(sh / kNumBoxes, SCMOS, kMinBox + sh % kNumBoxes, s->lanes[sh]);
       const int q = m.n_entries++;
       m.shape[q] = sh;
+      m.lanes[q] = L;
       m.first[q] = bp.start[sh];
       m.count[q] = bp.count[sh];
       m.tile_start[q] = tiles;
-      tiles += (bp.count[sh] + kMultiTile - 1) / kMultiTile;
+      tiles += (bp.count[sh] + kWave / L - 1) / (kWave / L);
     }
     m.tile_start[m.n_entries] = tiles;
     if ((e = set_kernel_node(sl.exec, sl.n_multi, multi_kernel_scmos(), (unsigned)std::max(1, tiles), kWave, &m)) !=
@@ -596,6 +605,7 @@ void dispatcher_main(gmle_service* s) {
     while (!s->pending.empty() && (int)sl.batch.size() < kMaxReqPerBatch) {
       const Pending& q = *s->pending.front();
       if (rois + q.n > s->cap_rois || pix + q.npix > s->cap_pix) break;
+      if (rois > 0 && rois + q.n > kTargetBatchRois) break;
       rois += q.n;
       pix += q.npix;
       sl.batch.push_back(s->pending.front());
@@ -637,12 +647,21 @@ void waiter_main(gmle_service* s, Slot* sl) {
     lock.unlock();
     // Poll rather than block: a blocking wait's wake-up costs more than a
     // small batch takes on the GPU.
-    hipError_t e;
-    while ((e = hipEventQuery(sl->start)) == hipErrorNotReady) std::this_thread::yield();
+    const double t_poll = now_s();
+    auto poll = [&](hipEvent_t ev) {
+      hipError_t r;
+      while ((r = hipEventQuery(ev)) == hipErrorNotReady) {
+        if (now_s() - t_poll < kSpinSeconds) {
+          std::this_thread::yield();
+        } else {
+          std::this_thread::sleep_for(std::chrono::microseconds(50));
+        }
+      }
+      return r;
+    };
+    hipError_t e = poll(sl->start);
     const double t_started = now_s();
-    if (e == hipSuccess) {
-      while ((e = hipEventQuery(sl->done)) == hipErrorNotReady) std::this_thread::yield();
-    }
+    if (e == hipSuccess) e = poll(sl->done);
     const double t_seen = now_s();
     float gpu_ms = 0.f;
     if (e == hipSuccess) (void)hipEventElapsedTime(&gpu_ms, sl->start, sl->done);
@@ -658,6 +677,8 @@ void waiter_main(gmle_service* s, Slot* sl) {
     s->stat_gpu += gpu_ms * 1e-3;
     s->stat_seen += t_seen - sl->t_launched;
     s->stat_start += t_started - sl->t_launched;
+    s->stat_gpu_max = std::max(s->stat_gpu_max, gpu_ms * 1e-3);
+    s->stat_seen_max = std::max(s->stat_seen_max, t_seen - sl->t_launched);
     lock.unlock();
     finish(s, b, e == hipSuccess ? GMLE_OK : GMLE_ERR_HIP);
     lock.lock();
@@ -919,6 +940,9 @@ int gmle_service_replay(gmle_service* s, int64_t n, const gmle_request* reqs, co
   const double t0 = now_s();
   for (int64_t i = 0; i < n; ++i) {
     const double due = t0 + arrival_s[i];
+    // sleep through long gaps, spin the last stretch
+    for (double left = due - now_s(); left > 3e-4; left = due - now_s())
+      std::this_thread::sleep_for(std::chrono::duration<double>(left - 2e-4));
     while (now_s() < due) {
     }
     PendingP& q = qs[(size_t)i];
@@ -963,9 +987,10 @@ int gmle_service_stats(gmle_service* s, double* out, int n) {
   if (!s || !out) return fail(GMLE_ERR_ARG, "NULL argument");
   std::lock_guard<std::mutex> lock(s->mu);
   const double b = s->stat_batches > 0 ? s->stat_batches : 1.0;
-  const double v[7] = {s->stat_batches, s->stat_rois / b, s->stat_reqs / b, s->stat_host / b * 1e6,
-                       s->stat_gpu / b * 1e6, s->stat_seen / b * 1e6, s->stat_start / b * 1e6};
-  for (int k = 0; k < n && k < 7; ++k) out[k] = v[k];
+  const double v[9] = {s->stat_batches, s->stat_rois / b, s->stat_reqs / b, s->stat_host / b * 1e6,
+                       s->stat_gpu / b * 1e6, s->stat_seen / b * 1e6, s->stat_start / b * 1e6,
+                       s->stat_gpu_max * 1e6, s->stat_seen_max * 1e6};
+  for (int k = 0; k < n && k < 9; ++k) out[k] = v[k];
   return GMLE_OK;
 }
 

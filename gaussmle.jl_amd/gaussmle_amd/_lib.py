@@ -55,6 +55,7 @@ SIGNATURES = {
          C.c_void_p, C.c_void_p],
     ),
     "gmle_shape_index": (C.c_int, [C.c_int, C.c_int]),
+    "gmle_one_launch_lanes": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int]),
     "gmle_batch_workspace_bytes": (C.c_size_t, [C.c_int64]),
     "gmle_fit_batch": (
         C.c_int,

@@ -136,10 +136,11 @@ int gmle_fit_device(int model, const float* psf, int box, int64_t n_rois, const 
  *                  lanes per ROI of each shape (0 = automatic), or NULL.
  *   streams        1-4: the per-shape launches are spread over this many HIP
  *                  streams so that shapes overlap on the GPU (1 = in order).
- *                  0: one launch for every shape, with 4 lanes per ROI (the
- *                  lanes table is then ignored); shapes whose 4-lane kernel
- *                  needs more than 256 VGPRs (astigmatic sCMOS ROIs of 14 px
- *                  and up) keep their own launch.  GMLE_FP32 only.
+ *                  0: one launch for every shape, each with its lanes from
+ *                  the table where that kernel fits 256 VGPRs (else 4 lanes);
+ *                  shapes whose 4-lane kernel needs more than 256 VGPRs
+ *                  (astigmatic sCMOS ROIs of 14 px and up) keep their own
+ *                  launch.  GMLE_FP32 only.
  * Every ROI's result is bit-identical to a gmle_fit call on that ROI alone with
  * the same model, box, lanes and precision.
  * ------------------------------------------------------------------------- */
@@ -147,6 +148,11 @@ int gmle_fit_device(int model, const float* psf, int box, int64_t n_rois, const 
 
 /* Index of (model, box) in the lanes / shape_counts tables, or -1. */
 int gmle_shape_index(int model, int box);
+
+/* Lanes per ROI a shape runs with in the one-launch mode (streams = 0) when
+ * `lanes` are requested (0 = automatic), or 0 when the shape gets a launch of
+ * its own there. */
+int gmle_one_launch_lanes(int model, int camera, int box, int lanes);
 
 /* Device workspace gmle_fit_batch_device needs for n_rois ROIs. */
 size_t gmle_batch_workspace_bytes(int64_t n_rois);
@@ -219,7 +225,8 @@ int gmle_host_free(void* p);
 /* A service on `device`.  hbm_budget: device bytes it may hold (maps plus
  * `slots` arenas of equal size); psf / lower / upper / max_step as
  * gmle_fit_batch; lanes (GMLE_NUM_SHAPES): lanes per ROI of each served shape,
- * 0 for shapes the service does not serve. */
+ * 0 for shapes the service does not serve (with GMLE_FP32 the served shapes
+ * run in one multi-shape kernel, as gmle_fit_batch with streams = 0). */
 int gmle_service_create(int device, size_t hbm_budget, int slots, int precision, int iterations,
                         const float* psf, const float* lower, const float* upper, const float* max_step,
                         const int32_t* lanes, gmle_service** out);
@@ -249,7 +256,9 @@ int gmle_service_info(gmle_service* s, int64_t* slot_rois, int64_t* slot_pixels,
 /* Per-batch averages since creation, up to n of: batches, ROIs per batch,
  * requests per batch, host time to form and launch a batch (us), GPU time of
  * a batch (us), time from launch to the completion being seen (us), time from
- * launch to the batch's start on the GPU being seen (us). */
+ * launch to the batch's start on the GPU being seen (us), largest GPU time of
+ * a batch (us), largest launch-to-completion time (us).  A batch stops
+ * growing at 65,536 ROIs (a larger request forms a batch of its own). */
 int gmle_service_stats(gmle_service* s, double* out, int n);
 
 /* Waits for in-flight work, then frees everything. */

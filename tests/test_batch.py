@@ -184,15 +184,24 @@ def test_batch_unknown_shapes_and_short_counts_give_nan():
 
 
 @pytest.mark.parametrize("name", ["scmos", "ideal"])
-def test_batch_one_launch_mode_matches_four_lane_plain(name):
-    """streams = 0: every shape in one multi-shape launch, 4 lanes per ROI."""
+@pytest.mark.parametrize("choice", ["default", "min", "max"])
+def test_batch_one_launch_mode_matches_plain(name, choice):
+    """streams = 0: every shape in one multi-shape launch, each with the
+    lanes gmle_one_launch_lanes reports; bit-identical to per-shape launches."""
+    lib = _lib.load()
     mb = mixed(name)
+    table = lanes_table(mb, choice)
     d = gb.upload_batch(mb)
-    out = gb.fit_batch_device(mb, d, "fp32", None, streams=0)
+    out = gb.fit_batch_device(mb, d, "fp32", table, streams=0)
     res = device_result(out, "fp32")
-    assert_same_as_plain(mb, res, "fp32", {k: 4 for k in resolved_lanes(mb, np.zeros(gb.NUM_SHAPES, np.int32),
-                                                                        "fp32")})
-    h = gb.fit_batch(mb, "fp32", None, streams=0)
+    used = {}
+    for s, c in enumerate(mb.shape_counts()):
+        if c:
+            m, b = gb.shape_of(s)
+            L = lib.gmle_one_launch_lanes(m, mb.camera, b, int(table[s]))
+            used[(m, b)] = L if L > 0 else resolved_lanes(mb, table, "fp32")[(m, b)]
+    print(f"[batch] one launch {name} {choice}: {used}", flush=True)
+    assert_same_as_plain(mb, res, "fp32", used)
+    h = gb.fit_batch(mb, "fp32", table, streams=0)
     np.testing.assert_array_equal(h.params, res["params"])
     np.testing.assert_array_equal(h.llr, res["llr"])
-

@@ -164,3 +164,23 @@ def test_frames_generator_layout():
     from gaussmle_amd import synth
     mu = synth.expected_counts(mb.psfs[0], mb.truth[:4, idx], 11)
     assert np.corrcoef(mb.roi_stack(idx, 11).ravel(), mu.ravel())[0, 1] > 0.95
+
+
+def test_one_launch_lanes(hiplib):
+    for m in range(4):
+        for cam in (0, 1):
+            for b in range(3, 17):
+                auto = hiplib.gmle_one_launch_lanes(m, cam, b, 0)
+                if m == 3 and cam == 1 and b >= 14:
+                    assert auto == 0          # own launch: no 4-lane tile within 256 VGPRs
+                    continue
+                assert auto in (1, 2, 4)
+                assert hiplib.gmle_one_launch_lanes(m, cam, b, 4) == 4
+                for want in (1, 2):
+                    got = hiplib.gmle_one_launch_lanes(m, cam, b, want)
+                    assert got in (want, 4)
+                    if got == want:   # a smaller box never loses a variant a larger one has
+                        assert all(hiplib.gmle_one_launch_lanes(m, cam, bb, want) == want for bb in range(3, b))
+    assert hiplib.gmle_one_launch_lanes(0, 0, 11, 1) == 1
+    assert hiplib.gmle_one_launch_lanes(9, 0, 11, 1) == 0
+
