@@ -366,20 +366,16 @@ int enqueue_batch(const BatchSpec& b, hipStream_t stream) {
       a.offsets = b.offsets;
       a.out_stride = kBatchStride;
     }
-    int32_t tiles = 0;
     for (int s : multi) {
       const int model = s / kNumBoxes, box = kMinBox + s % kNumBoxes;
       const int want = (b.lanes && b.lanes[s] > 0) ? b.lanes[s] : large_batch_lanes(model, b.camera, box);
-      const int L = multi_lanes(model, b.camera, box, want);
       const int q = m.n_entries++;
       m.shape[q] = s;
-      m.lanes[q] = L;
+      m.lanes[q] = multi_lanes(model, b.camera, box, want);
       m.first[q] = p.start[s];
       m.count[q] = p.count[s];
-      m.tile_start[q] = tiles;
-      tiles += (p.count[s] + kWave / L - 1) / (kWave / L);
     }
-    m.tile_start[m.n_entries] = tiles;
+    multi_plan_blocks(m, kMultiTargetBlocks);
     if (m.n_entries > 0) {
       e = b.camera == SCMOS ? launch_multi_scmos(m, stream) : launch_multi_ideal(m, stream);
       if (e != hipSuccess) return hip_fail(e, "kernel launch");

@@ -534,20 +534,17 @@ int launch_slot(gmle_service* s, Slot& sl) {
       n.model = model;
       m.fa[model] = shape_args(s, sl, n, 0, 0);
     }
-    int32_t tiles = 0;
     for (int sh : sl.multi_shapes) {   // heaviest shapes' tiles first
       if (counts[sh] == 0) continue;
-      const int L = multi_lanes(sh / kNumBoxes, SCMOS, kMinBox + sh % kNumBoxes, s->lanes[sh]);
       const int q = m.n_entries++;
       m.shape[q] = sh;
-      m.lanes[q] = L;
+      m.lanes[q] = multi_lanes(sh / kNumBoxes, SCMOS, kMinBox + sh % kNumBoxes, s->lanes[sh]);
       m.first[q] = bp.start[sh];
       m.count[q] = bp.count[sh];
-      m.tile_start[q] = tiles;
-      tiles += (bp.count[sh] + kWave / L - 1) / (kWave / L);
     }
-    m.tile_start[m.n_entries] = tiles;
-    if ((e = set_kernel_node(sl.exec, sl.n_multi, multi_kernel_scmos(), (unsigned)std::max(1, tiles), kWave, &m)) !=
+    multi_plan_blocks(m, kMultiTargetBlocks);
+    const int32_t blocks = m.block_start[m.n_entries];
+    if ((e = set_kernel_node(sl.exec, sl.n_multi, multi_kernel_scmos(), (unsigned)std::max(1, blocks), kWave, &m)) !=
         hipSuccess)
       return hip_fail(e, "hipGraphExecKernelNodeSetParams");
   }

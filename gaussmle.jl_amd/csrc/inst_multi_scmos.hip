@@ -2,9 +2,9 @@
 #include "gmle_multi.hpp"
 namespace gmle {
 hipError_t launch_multi_scmos(const MultiArgs& m, hipStream_t stream) {
-  const int32_t tiles = m.tile_start[m.n_entries];
-  if (tiles <= 0) return hipSuccess;
-  hipLaunchKernelGGL((multi_fit<SCMOS>), dim3((unsigned)tiles), dim3(kWave), 0, stream, m);
+  const int32_t blocks = m.block_start[m.n_entries];
+  if (blocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL((multi_fit<SCMOS>), dim3((unsigned)blocks), dim3(kWave), 0, stream, m);
   return hipGetLastError();
 }
 const void* multi_kernel_scmos() { return reinterpret_cast<const void*>(&multi_fit<SCMOS>); }

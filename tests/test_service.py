@@ -38,12 +38,12 @@ def service():
 
 def reference(s, frs):
     """The batch path on host-converted electrons, with the lanes the service
-    uses: Float32 services fit every shape here with the multi-shape kernel
-    (4 lanes per ROI), mixed-precision ones with the lanes table."""
+    uses: Float32 services fit every shape in one multi-shape kernel (the
+    batch path's one-launch mode), mixed-precision ones per shape."""
     maps, _ = frames()
     mb = sv.frames_batch(frs, maps, s.psfs, s.constraints, s.iterations)
     if s.precision == "fp32":
-        return gb.fit_batch(mb, "fp32", 4, streams=2)
+        return gb.fit_batch(mb, "fp32", s.table, streams=0)
     return gb.fit_batch(mb, s.precision, s.table, streams=2)
 
 
@@ -78,7 +78,7 @@ def test_service_matches_batch_path_alone_and_in_mix(service):
                 assert_same(r.results(), want[i], f"mix round {rnd} request {i}")
             if rnd == 1:
                 again = gb.fit_batch(sv.frames_batch(frs[:2], maps, service.psfs, service.constraints), "fp32",
-                                     4, streams=1)
+                                     service.table, streams=0)
                 np.testing.assert_array_equal(again.params, np.concatenate([w[0] for w in want[:2]], axis=1))
         # replayed at a rate: arrivals 0.2 ms apart
         lat, el = service.replay(reqs, np.arange(len(reqs)) * 2e-4)
