@@ -375,7 +375,7 @@ int enqueue_batch(const BatchSpec& b, hipStream_t stream) {
       m.first[q] = p.start[s];
       m.count[q] = p.count[s];
     }
-    multi_plan_blocks(m, kMultiTargetBlocks);
+    multi_plan_blocks(m);
     if (m.n_entries > 0) {
       e = b.camera == SCMOS ? launch_multi_scmos(m, stream) : launch_multi_ideal(m, stream);
       if (e != hipSuccess) return hip_fail(e, "kernel launch");

@@ -37,54 +37,48 @@ struct MultiArgs {
   int32_t lanes[kNumShapes];      //          lanes per ROI (multi_has)
   int32_t first[kNumShapes];      //          its first slot in order[]
   int32_t count[kNumShapes];      //          its slots
-  int32_t block_start[kNumShapes + 1];   // first block of entry q; total blocks at n_entries
-  int32_t tiles_per_block;        // consecutive tiles of one shape per block (>= 1)
+  int32_t block_start[kNumShapes + 1];   // first block (= tile) of entry q; total at n_entries
 };
 
-// Tiles [t0, t1) of one shape.  Several tiles per call amortise the call's
-// register saves over more work.
+// Tile t of one shape.  One tile per call and per block: a loop over tiles,
+// in here or around the call, keeps state live across fit_one and costs
+// registers (the kernel would drop to one wave per SIMD).
 template <int MODEL, int CAM, int N, int L>
-__device__ __attribute__((noinline)) void multi_tile(const FitArgs& a, int32_t first, int32_t count, int32_t t0,
-                                                     int32_t t1) {
+__device__ __attribute__((noinline)) void multi_tile(const FitArgs& a, int32_t first, int32_t count, int32_t t) {
   constexpr int RPT = kWave / L;   // ROIs per tile
   const int lane = threadIdx.x;
   const int r = (L == 1) ? 0 : (lane & (L - 1));
   // An entry whose first slot is empty holds no ROI of this shape.
   const int32_t k0 = __builtin_amdgcn_readfirstlane(a.order[first]);
   if (k0 < 0) return;
-  for (int32_t t = t0; t < t1; ++t) {
-    const int32_t j = t * RPT + lane / L;
-    const bool in = j < count;
-    int32_t k = a.order[first + (in ? j : 0)];
-    const bool active = in && k >= 0;
-    if (k < 0) k = k0;
-    RoiRef ref;
-    ref.data = a.data + a.offsets[k];
-    ref.out = k;
-    ref.stride = kBatchStride;
-    ref.cx = ref.cy = 1;
-    if constexpr (CAM == SCMOS) {
-      ref.cx = a.x_corners[k];
-      ref.cy = a.y_corners[k];
-    }
-    fit_one<MODEL, CAM, N, L, PREC_FP32>(a, ref, active, r);
+  const int32_t j = t * RPT + lane / L;
+  const bool in = j < count;
+  int32_t k = a.order[first + (in ? j : 0)];
+  const bool active = in && k >= 0;
+  if (k < 0) k = k0;
+  RoiRef ref;
+  ref.data = a.data + a.offsets[k];
+  ref.out = k;
+  ref.stride = kBatchStride;
+  ref.cx = ref.cy = 1;
+  if constexpr (CAM == SCMOS) {
+    ref.cx = a.x_corners[k];
+    ref.cy = a.y_corners[k];
   }
+  fit_one<MODEL, CAM, N, L, PREC_FP32>(a, ref, active, r);
 }
 
 template <int CAM, int MODEL, int N>
-__device__ __forceinline__ void multi_case(const MultiArgs& m, int q, int b) {
+__device__ __forceinline__ void multi_case(const MultiArgs& m, int q, int t) {
   const int L = m.lanes[q];
-  const int32_t tiles = (m.count[q] + kWave / L - 1) / (kWave / L);
-  const int32_t t0 = b * m.tiles_per_block;
-  const int32_t t1 = min(t0 + m.tiles_per_block, tiles);
   if constexpr (multi_has(MODEL, CAM, N, 1)) {
-    if (L == 1) { multi_tile<MODEL, CAM, N, 1>(m.fa[MODEL], m.first[q], m.count[q], t0, t1); return; }
+    if (L == 1) { multi_tile<MODEL, CAM, N, 1>(m.fa[MODEL], m.first[q], m.count[q], t); return; }
   }
   if constexpr (multi_has(MODEL, CAM, N, 2)) {
-    if (L == 2) { multi_tile<MODEL, CAM, N, 2>(m.fa[MODEL], m.first[q], m.count[q], t0, t1); return; }
+    if (L == 2) { multi_tile<MODEL, CAM, N, 2>(m.fa[MODEL], m.first[q], m.count[q], t); return; }
   }
   if constexpr (multi_has(MODEL, CAM, N, 4)) {
-    if (L == 4) multi_tile<MODEL, CAM, N, 4>(m.fa[MODEL], m.first[q], m.count[q], t0, t1);
+    if (L == 4) multi_tile<MODEL, CAM, N, 4>(m.fa[MODEL], m.first[q], m.count[q], t);
   }
 }
 
@@ -109,8 +103,7 @@ __device__ __forceinline__ void multi_box(const MultiArgs& m, int box, int q, in
   }
 }
 
-// One block (one wave) per tiles_per_block tiles of one shape; blocks of the
-// heaviest shapes come first.
+// One block (one wave) per tile; tiles of the heaviest shapes come first.
 template <int CAM>
 __global__ __launch_bounds__(kWave) void multi_fit(MultiArgs m) {
   const int32_t blk = blockIdx.x;
@@ -128,26 +121,15 @@ __global__ __launch_bounds__(kWave) void multi_fit(MultiArgs m) {
   }
 }
 
-// Fills lanes' tiles into block_start with tiles_per_block chosen so that the
-// launch has about `target_blocks` blocks (at least one tile per block).
-inline void multi_plan_blocks(MultiArgs& m, int64_t target_blocks) {
-  int64_t tiles = 0;
-  for (int q = 0; q < m.n_entries; ++q) tiles += (m.count[q] + kWave / m.lanes[q] - 1) / (kWave / m.lanes[q]);
-  const int64_t k = target_blocks > 0 ? (tiles + target_blocks - 1) / target_blocks : 1;
-  m.tiles_per_block = (int32_t)(k < 1 ? 1 : k);
+// Fills block_start from the entries' counts and lanes (one tile per block).
+inline void multi_plan_blocks(MultiArgs& m) {
   int32_t blocks = 0;
   for (int q = 0; q < m.n_entries; ++q) {
-    const int32_t t = (m.count[q] + kWave / m.lanes[q] - 1) / (kWave / m.lanes[q]);
     m.block_start[q] = blocks;
-    blocks += (t + m.tiles_per_block - 1) / m.tiles_per_block;
+    blocks += (m.count[q] + kWave / m.lanes[q] - 1) / (kWave / m.lanes[q]);
   }
   m.block_start[m.n_entries] = blocks;
 }
-
-// Blocks of a multi-shape launch: enough to fill the chip several times over
-// (256 CUs x 4 SIMDs x 2 waves), so load balances while each block still
-// amortises its calls over several tiles in large batches.
-constexpr int64_t kMultiTargetBlocks = 256 * 4 * 2 * 8;
 
 // Launcher and kernel address of one camera's multi-shape kernel.
 hipError_t launch_multi_ideal(const MultiArgs& m, hipStream_t stream);

@@ -542,7 +542,7 @@ int launch_slot(gmle_service* s, Slot& sl) {
       m.first[q] = bp.start[sh];
       m.count[q] = bp.count[sh];
     }
-    multi_plan_blocks(m, kMultiTargetBlocks);
+    multi_plan_blocks(m);
     const int32_t blocks = m.block_start[m.n_entries];
     if ((e = set_kernel_node(sl.exec, sl.n_multi, multi_kernel_scmos(), (unsigned)std::max(1, blocks), kWave, &m)) !=
         hipSuccess)
