@@ -43,7 +43,7 @@
 namespace gmle {
 namespace {
 
-constexpr int kChunk = 64;          // ROIs per ingest / result block
+constexpr int kChunk = 16;          // ROIs per ingest / result block
 constexpr int kIngestBlock = 256;
 constexpr int kMaxReqPerBatch = 4096;
 // A batch stops growing at this many ROIs (a larger request still forms a
@@ -137,13 +137,15 @@ struct ResultArgs {
   const float* llr;
 };
 
-// One block per chunk: per-ROI metadata and a box^2 prefix in LDS, then the
-// chunk's pixels (contiguous in the request) converted to electrons.
+// One block per chunk of kChunk ROIs: lanes t < count load the ROIs'
+// metadata (box sizes clamped, so a request rewritten after submission can
+// never move a write outside its slot), then each wave converts whole ROIs
+// (wave w takes ROIs w, w + 4, ...), all of an ROI's pixels in flight at once.
 __global__ __launch_bounds__(kIngestBlock) void service_ingest(IngestArgs a) {
-  __shared__ int pref[kChunk + 1];
   __shared__ int sbox[kChunk];
   __shared__ int sxc[kChunk];
   __shared__ int syc[kChunk];
+  __shared__ int spre[kChunk + 1];
   const Chunk c = a.chunks[blockIdx.x];
   const ReqDesc r = a.reqs[c.req];
   const int t = threadIdx.x;
@@ -151,40 +153,44 @@ __global__ __launch_bounds__(kIngestBlock) void service_ingest(IngestArgs a) {
     const int k = c.first + t;
     const int64_t roi = r.roi_base + k;
     const int b = min(max((int)r.boxes[k], kMinBox), kMaxBox);
+    const int x = r.xc[k], y = r.yc[k];
     sbox[t] = b;
-    sxc[t] = r.xc[k];
-    syc[t] = r.yc[k];
+    sxc[t] = x;
+    syc[t] = y;
     a.models[roi] = r.models[k];
     a.boxes[roi] = (int8_t)b;
-    a.xc[roi] = sxc[t];
-    a.yc[roi] = syc[t];
+    a.xc[roi] = x;
+    a.yc[roi] = y;
   }
   __syncthreads();
   if (t == 0) {
-    int s = 0;
+    int sum = 0;
     for (int q = 0; q < c.count; ++q) {
-      pref[q] = s;
-      s += sbox[q] * sbox[q];
+      spre[q] = sum;
+      sum += sbox[q] * sbox[q];
     }
-    pref[c.count] = s;
+    spre[c.count] = sum;
   }
   __syncthreads();
-  if (t < c.count) a.offsets[r.roi_base + c.first + t] = r.pix_base + c.pix0 + pref[t];
-  const int total = min(pref[c.count], c.npix);
-  const uint16_t* src = r.adu + c.pix0;
-  float* dst = a.data + r.pix_base + c.pix0;
-  for (int p = t; p < total; p += kIngestBlock) {
-    int lo = 0, hi = c.count - 1;   // ROI q with pref[q] <= p < pref[q + 1]
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (pref[mid] <= p) lo = mid; else hi = mid - 1;
+  if (t < c.count) a.offsets[r.roi_base + c.first + t] = r.pix_base + c.pix0 + spre[t];
+  const int wave = t / kWave, lane = t % kWave;
+  for (int q = wave; q < c.count; q += kIngestBlock / kWave) {
+    const int b = sbox[q];
+    const int p0 = spre[q];
+    if (p0 + b * b > c.npix) break;   // validated sizes never get here
+    const uint16_t* src = r.adu + c.pix0 + p0;
+    float* dst = a.data + r.pix_base + c.pix0 + p0;
+    const int64_t m0 = (int64_t)(syc[q] - 1) + (int64_t)a.ny * (int64_t)(sxc[q] - 1);
+#pragma unroll
+    for (int u = 0; u < (kMaxBox * kMaxBox + kWave - 1) / kWave; ++u) {
+      const int p = lane + u * kWave;
+      if (p < b * b) {
+        const int i = p % b, j = p / b;   // row, column inside the ROI
+        int64_t m = m0 + i + (int64_t)a.ny * j;
+        m = m < 0 ? 0 : (m >= a.map_size ? a.map_size - 1 : m);   // corners are validated; never read outside
+        dst[p] = ((float)src[p] - a.offset_map[m]) * a.gain_map[m];
+      }
     }
-    const int b = sbox[lo];
-    const int q = p - pref[lo];
-    const int i = q % b, j = q / b;   // row, column inside the ROI
-    int64_t m = (int64_t)(syc[lo] - 1 + i) + (int64_t)a.ny * (int64_t)(sxc[lo] - 1 + j);
-    m = m < 0 ? 0 : (m >= a.map_size ? a.map_size - 1 : m);   // corners are validated; never read outside
-    dst[p] = ((float)src[p] - a.offset_map[m]) * a.gain_map[m];
   }
 }
 
