@@ -197,6 +197,19 @@ def run_service(n_frames: int = 300, submissions: int = 3000, loads=(0.1, 0.5, 0
         res["batch_stats"] = svc.stats()
     for r in reqs:
         r.close()
+    # PCIe host-to-device rate of one large pinned copy: the ceiling for any
+    # path that must move the frames' raw counts to the GPU
+    src = torch.empty(256 << 20, dtype=torch.uint8).pin_memory()
+    dst = torch.empty(256 << 20, dtype=torch.uint8, device="cuda:0")
+    dst.copy_(src, non_blocking=True)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(5):
+        dst.copy_(src, non_blocking=True)
+    torch.cuda.synchronize()
+    res["pcie_h2d_gbs"] = round(5 * (256 << 20) / (time.perf_counter() - t1) / 1e9, 2)
+    res["raw_bytes_per_roi"] = round(float(sum(f.adu.nbytes for f in frames)) / rois.sum(), 1)
+    del src, dst
     # the plain batch kernel on the same ROIs (electrons, device-resident)
     mb = sv.frames_batch(frames, maps, psfs, cons)
     d = gb.upload_batch(mb)
@@ -208,6 +221,12 @@ def run_service(n_frames: int = 300, submissions: int = 3000, loads=(0.1, 0.5, 0
         out = gb.fit_batch_device(mb, d, "fp32", t, 4, out=out)
     torch.cuda.synchronize()
     res["plain_batch_fits_per_s"] = round(5 * mb.n / (time.perf_counter() - t1), 1)
+    # the same with host arrays: upload of Float32 electrons, fit, download
+    gb.fit_batch(mb, "fp32", t, 4)
+    t1 = time.perf_counter()
+    for _ in range(3):
+        gb.fit_batch(mb, "fp32", t, 4)
+    res["plain_host_path_fits_per_s"] = round(3 * mb.n / (time.perf_counter() - t1), 1)
     return res
 
 

@@ -24,6 +24,7 @@
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -218,6 +219,7 @@ struct Pending {
   const int32_t* xc = nullptr;
   const int32_t* yc = nullptr;
   const uint16_t* adu = nullptr;   // host address (pinned)
+  const uint16_t* adu_dev = nullptr;   // its device address
   float* params = nullptr;
   float* crlb = nullptr;
   float* llr = nullptr;
@@ -310,6 +312,7 @@ struct gmle_service {
   double stat_batches = 0, stat_rois = 0, stat_reqs = 0, stat_host = 0, stat_gpu = 0, stat_seen = 0, stat_start = 0;
   double stat_gpu_max = 0, stat_seen_max = 0;
   int launch_error = GMLE_OK;
+  bool direct_ingest = false;   // ingest reads request counts from pinned memory (no DMA copies)
   std::string error_msg;        // first failure of a batch (launch or completion)
 };
 
@@ -498,7 +501,7 @@ int launch_slot(gmle_service* s, Slot& sl) {
     d.boxes = q.boxes;
     d.xc = q.xc;
     d.yc = q.yc;
-    d.adu = sl.raw + pix;
+    d.adu = s->direct_ingest ? q.adu_dev : sl.raw + pix;
     d.params = q.params;
     d.crlb = q.crlb;
     d.llr = q.llr;
@@ -567,8 +570,8 @@ int launch_slot(gmle_service* s, Slot& sl) {
     }
   }
   if ((e = hipEventRecord(sl.start, sl.stream)) != hipSuccess) return hip_fail(e, "hipEventRecord");
-  // Raw counts over PCIe by DMA, then the graph.
-  for (size_t r = 0; r < sl.batch.size(); ++r) {
+  // Raw counts over PCIe by DMA (unless ingest reads them in place), then the graph.
+  for (size_t r = 0; !s->direct_ingest && r < sl.batch.size(); ++r) {
     const Pending& q = *sl.batch[r];
     if ((e = hipMemcpyAsync(sl.raw + (sl.reqs_h[r].pix_base), q.adu, (size_t)q.npix * sizeof(uint16_t),
                             hipMemcpyHostToDevice, sl.stream)) != hipSuccess)
@@ -729,6 +732,7 @@ int make_pending(gmle_service* s, const gmle_request* r, PendingP& out) {
   q->npix = npix;
   if (!pinned_dev(r->adu, 2 * npix, &d)) return fail(GMLE_ERR_ARG, "request pixels are not in gmle_host_alloc memory");
   q->adu = r->adu;
+  q->adu_dev = (const uint16_t*)d;
   if (!pinned_dev(r->params, 4 * kBatchStride * n, &d)) return fail(GMLE_ERR_ARG, "params not in gmle_host_alloc memory");
   q->params = (float*)d;
   if (!pinned_dev(r->crlb, 4 * kBatchStride * n, &d)) return fail(GMLE_ERR_ARG, "crlb not in gmle_host_alloc memory");
@@ -858,6 +862,11 @@ int gmle_service_create(int device, size_t hbm_budget, int slots, int precision,
   std::memcpy(s->lanes, lanes, sizeof(s->lanes));
   s->budget = hbm_budget;
   s->max_box = max_box;
+  // How raw counts reach the GPU: "direct" (default: the ingest kernel reads
+  // them from the pinned request buffers over PCIe; lower latency) or "dma"
+  // (one copy per request into the slot before its graph).
+  const char* ingest = std::getenv("GMLE_SERVICE_INGEST");
+  s->direct_ingest = !(ingest && std::string(ingest) == "dma");
   (void)hipSetDevice(device);
   s->slots.clear();
   for (int k = 0; k < slots; ++k) {
