@@ -54,6 +54,10 @@ SIGNATURES = {
          C.c_void_p, _F, _F, _F, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
          C.c_void_p, C.c_void_p],
     ),
+    "gmle_epilogue_device": (C.c_int, [C.c_int64, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                       C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double,
+                                       C.c_void_p, C.c_void_p]),
+    "gmle_chi2_pvalue": (C.c_float, [C.c_float, C.c_int]),
     "gmle_shape_index": (C.c_int, [C.c_int, C.c_int]),
     "gmle_one_launch_lanes": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int]),
     "gmle_batch_workspace_bytes": (C.c_size_t, [C.c_int64]),

@@ -60,3 +60,32 @@ def upload_workload(w, device="cuda:0"):
         d["xc"] = torch.from_numpy(w.x_corners).to(device)
         d["yc"] = torch.from_numpy(w.y_corners).to(device)
     return d
+
+
+EPILOGUE_FIELDS = ("x_um", "y_um", "crlb_x_um", "crlb_y_um", "pvalue", "width_x_um", "width_y_um",
+                   "crlb_width_x_um", "crlb_width_y_um", "chi2")
+
+
+def epilogue_device(d_params, d_crlb, d_llr, pixel_size_x: float, pixel_size_y: float | None = None, model: int = 0,
+                    box: int = 11, d_models=None, d_boxes=None, d_xc=None, d_yc=None, stream=None, out=None):
+    """gmle_epilogue_device on torch tensors: (n, 10) Float32 per-emitter
+    results (EPILOGUE_FIELDS) in microns with p-values, enqueued on torch's
+    current stream.  d_params / d_crlb are (n, stride)."""
+    import torch
+
+    lib = _lib.load()
+    n = d_llr.numel()
+    stride = d_params.shape[1]
+    if out is None:
+        out = torch.empty((n, len(EPILOGUE_FIELDS)), dtype=torch.float32, device=d_llr.device)
+    s = stream if stream is not None else torch.cuda.current_stream(d_llr.device)
+
+    def ptr(t):
+        return None if t is None else t.data_ptr()
+
+    rc = lib.gmle_epilogue_device(n, model, box, ptr(d_models), ptr(d_boxes), stride, ptr(d_params), ptr(d_crlb),
+                                  ptr(d_llr), ptr(d_xc), ptr(d_yc), float(pixel_size_x),
+                                  float(pixel_size_x if pixel_size_y is None else pixel_size_y), ptr(out),
+                                  s.cuda_stream)
+    _lib.check(rc)
+    return out

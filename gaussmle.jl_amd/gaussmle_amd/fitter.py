@@ -204,7 +204,7 @@ def fit(fitter: GaussMLEFitter, data, variance_map: np.ndarray | None = None):
     else:
         out = fit_arrays(fitter, arr, psf_pixels, np.asarray(variance_map, dtype=F32), xc, yc)
     pv = pvalues(out.llr, box, len(fitter.psf_model))
-    camera = IdealCamera.square(1024, 1024, pixel_size)
+    camera = IdealCamera.square(1024, 1024, pixel_size, dtype=np.float32)   # 0.1f0 (src/interface.jl:191,284)
     batch = ROIBatch(np.asarray(arr, dtype=F32), xc, yc, np.ones(n, dtype=np.int32), camera)
     loc = LocalizationResult.create(out, pv, batch, fitter.psf_model)
     return to_smld(loc, batch)

@@ -156,8 +156,10 @@ class IdealCamera:
     pixel_edges_y: np.ndarray
 
     @classmethod
-    def square(cls, nx: int, ny: int, pixel_size: float) -> "IdealCamera":
-        return cls(np.arange(nx + 1, dtype=np.float64) * pixel_size, np.arange(ny + 1, dtype=np.float64) * pixel_size)
+    def square(cls, nx: int, ny: int, pixel_size: float, dtype=np.float64) -> "IdealCamera":
+        """Edges 0, ps, 2 ps, ... in `dtype` (the reference's pixel size type)."""
+        ps = np.asarray(pixel_size, dtype=dtype)
+        return cls(np.arange(nx + 1, dtype=dtype) * ps, np.arange(ny + 1, dtype=dtype) * ps)
 
     @property
     def pixel_size(self) -> float:

@@ -131,31 +131,32 @@ class BasicSMLD:
 
 def to_emitter(result: LocalizationResult, idx: int, camera, dataset: int = 1, track_id: int = 0, id: int | None = None):
     """src/roi_batch.jl:121-302 (idx is 0-based here; id defaults to idx + 1)."""
-    psx = F32(camera.pixel_edges_x[1] - camera.pixel_edges_x[0])
-    psy = F32(camera.pixel_edges_y[1] - camera.pixel_edges_y[0])
+    # pixel size in the edges' own type: Float32 values times a Float64 pixel
+    # size are computed in Float64 and rounded to Float32 once (T(...))
+    psx = camera.pixel_edges_x[1] - camera.pixel_edges_x[0]
+    psy = camera.pixel_edges_y[1] - camera.pixel_edges_y[0]
     p = result.parameters[:, idx]
     u = result.uncertainties[:, idx]
-    x_um = float((result.x_camera[idx] - F32(1)) * psx)
-    y_um = float((result.y_camera[idx] - F32(1)) * psy)
+    x_um = float(F32((result.x_camera[idx] - F32(1)) * psx))
+    y_um = float(F32((result.y_camera[idx] - F32(1)) * psy))
     frame = int(result.frame_indices[idx])
     pv = float(result.pvalues[idx])
     eid = idx + 1 if id is None else id
     m = result.psf_model
+    sx = lambda v: float(F32(v * psx))  # noqa: E731
+    sy = lambda v: float(F32(v * psy))  # noqa: E731
     if isinstance(m, GaussianXYNB):
-        return Emitter2DFitGaussMLE(x_um, y_um, float(p[2]), float(p[3]), float(u[0] * psx), float(u[1] * psy),
-                                    float(u[2]), float(u[3]), pv, frame, dataset, track_id, eid)
+        return Emitter2DFitGaussMLE(x_um, y_um, float(p[2]), float(p[3]), sx(u[0]), sy(u[1]), float(u[2]),
+                                    float(u[3]), pv, frame, dataset, track_id, eid)
     if isinstance(m, GaussianXYNBS):
-        return Emitter2DFitSigma(x_um, y_um, float(p[2]), float(p[3]), float(p[4] * psx), float(u[0] * psx),
-                                 float(u[1] * psy), float(u[2]), float(u[3]), float(u[4] * psx), pv, frame, dataset,
-                                 track_id, eid)
+        return Emitter2DFitSigma(x_um, y_um, float(p[2]), float(p[3]), sx(p[4]), sx(u[0]), sy(u[1]), float(u[2]),
+                                 float(u[3]), sx(u[4]), pv, frame, dataset, track_id, eid)
     if isinstance(m, GaussianXYNBSXSY):
-        return Emitter2DFitSigmaXY(x_um, y_um, float(p[2]), float(p[3]), float(p[4] * psx), float(p[5] * psy),
-                                   float(u[0] * psx), float(u[1] * psy), float(u[2]), float(u[3]), float(u[4] * psx),
-                                   float(u[5] * psy), pv, frame, dataset, track_id, eid)
+        return Emitter2DFitSigmaXY(x_um, y_um, float(p[2]), float(p[3]), sx(p[4]), sy(p[5]), sx(u[0]), sy(u[1]),
+                                   float(u[2]), float(u[3]), sx(u[4]), sy(u[5]), pv, frame, dataset, track_id, eid)
     if isinstance(m, AstigmaticXYZNB):
-        return Emitter3DFitGaussMLE(x_um, y_um, float(p[2]), float(p[3]), float(p[4]), float(u[0] * psx),
-                                    float(u[1] * psy), float(u[2]), float(u[3]), float(u[4]), pv, frame, dataset,
-                                    track_id, eid)
+        return Emitter3DFitGaussMLE(x_um, y_um, float(p[2]), float(p[3]), float(p[4]), sx(u[0]), sy(u[1]),
+                                    float(u[2]), float(u[3]), float(u[4]), pv, frame, dataset, track_id, eid)
     raise TypeError(f"unknown PSF model {m!r}")
 
 

@@ -167,4 +167,31 @@ end
 
 hip_service_destroy(s) = check(ccall((:gmle_service_destroy, libgmle), Cint, (Ptr{Cvoid},), s))
 
+"""
+    hip_chi2_pvalue(chi2, df)
+
+`1 - Float32(cdf(Chisq(df), chi2))` as the device epilogue computes it
+(src/interface.jl:265-275), for callers that finish fits on the host.
+"""
+hip_chi2_pvalue(chi2::Real, df::Integer) = ccall((:gmle_chi2_pvalue, libgmle), Cfloat, (Cfloat, Cint), chi2, df)
+
+"""
+    hip_epilogue!(out, params, crlb, llr, model, box, stride, x_corners, y_corners, psx, psy, n; stream = C_NULL)
+
+Device arrays (`CuArray`-like pointers from a HIP array package, or raw
+`Ptr`s): `out` is (10, n) Float32 with x, y, σx, σy in µm, the p-value,
+fitted widths and their uncertainties in µm, and χ² per ROI
+(src/roi_batch.jl:29-34, :125-302).  `params` / `crlb` are (stride, n);
+C_NULL corners mean fit(Array)'s x = 1 + (k-1) box, y = 1.
+"""
+function hip_epilogue!(out, params, crlb, llr, model::Integer, box::Integer, stride::Integer, x_corners, y_corners,
+                       psx::Real, psy::Real, n::Integer; stream = C_NULL)
+    check(ccall((:gmle_epilogue_device, libgmle), Cint,
+                (Int64, Cint, Cint, Ptr{Int8}, Ptr{Int8}, Cint, Ptr{Cfloat}, Ptr{Cfloat}, Ptr{Cfloat},
+                 Ptr{Int32}, Ptr{Int32}, Cdouble, Cdouble, Ptr{Cfloat}, Ptr{Cvoid}),
+                n, model, box, C_NULL, C_NULL, stride, params, crlb, llr, x_corners, y_corners, psx, psy, out,
+                stream))
+    return out
+end
+
 end # module

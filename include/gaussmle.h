@@ -181,6 +181,31 @@ int gmle_fit_batch_device(int64_t n_rois, const int8_t* d_models, const int8_t* 
                           void* d_workspace, size_t workspace_bytes, void* stream);
 
 /* ---------------------------------------------------------------------------
+ * Result epilogue on the device: what GaussMLE.jl does per emitter on the host
+ * after the fit (src/interface.jl:265-275, src/roi_batch.jl:29-34, 125-302).
+ * For each ROI k, out[10*k + q] holds, in Float32:
+ *   0, 1  x, y in microns: ((corner + roi coordinate - 1) - 1) * pixel size
+ *   2, 3  their CRLB in microns
+ *   4     p-value: 1 - Float32(cdf(Chisq(box^2 - P), -2 LLR))
+ *   5, 6  fitted widths in microns (XYNBS: sigma in both; XYNBSXSY: sigma_x,
+ *         sigma_y; NaN otherwise)
+ *   7, 8  their CRLB in microns (NaN where 5, 6 are)
+ *   9     chi^2 = -2 LLR
+ * Photons, background, z and their CRLB are used as fitted (unscaled).
+ * model / box apply to every ROI unless d_models / d_boxes (per ROI) are
+ * given; stride is the params / crlb values per ROI (P, or 6 for mixed
+ * batches).  Without corners, ROI k sits at x = 1 + k*box, y = 1, as in
+ * fit(fitter, Array) (src/interface.jl:203-205).  Enqueued on `stream`.
+ * ------------------------------------------------------------------------- */
+int gmle_epilogue_device(int64_t n_rois, int model, int box, const int8_t* d_models, const int8_t* d_boxes,
+                         int stride, const float* d_params, const float* d_crlb, const float* d_llr,
+                         const int32_t* d_x_corners, const int32_t* d_y_corners, double pixel_size_x,
+                         double pixel_size_y, float* d_out, void* stream);
+
+/* The epilogue's p-value on the host: 1 - Float32(cdf(Chisq(df), chi2)). */
+float gmle_chi2_pvalue(float chi2, int df);
+
+/* ---------------------------------------------------------------------------
  * Serving: a resident fitting service with continuous batching.
  *
  * The reference is a library call per batch: upload, launch, wait, download
