@@ -30,6 +30,7 @@ METRIC = "ROI fits/sec (11×11 XYNB, 20 iters) plain vs batched; p99 latency und
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP32_UNPACKED_TFLOPS = 78.6   # FP32 vector without packed (2-wide) instructions
 PEAK_HBM_GBS = 8000.0
+VALU_LANE_OPS_PER_S = 256 * 4 * 16 * 2.4e9   # CUs x SIMDs x lanes x clock: one FP32 op per lane-cycle
 # GPU fits/s quoted by the reference for 11x11 GaussianXYNB (RTX 4090, batch 50K).
 REFERENCE_GPU_FITS_PER_S = 1.0e7
 
@@ -266,6 +267,16 @@ def host_cpu() -> dict:
     return {"cpu_model": model, "nproc": usable, "cpu_count": os.cpu_count()}
 
 
+def instruction_mix(workload: str):
+    """VALU / transcendental instructions per fit measured offline with
+    rocprofv3 --pmc (SQ_INSTS_VALU, SQ_INSTS_VALU_TRANS_F32), if recorded."""
+    p = ROOT / "profiles" / f"traffic_{workload}.json"
+    try:
+        return json.loads(p.read_text())["instruction_mix_per_fit"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(w, sample: int, threads: int, runs: int = 5):
     """Time the oracle (the reference's algorithm restated in C++, Float32) on
     the first `sample` ROIs with `threads` OpenMP threads: one warm-up run,
@@ -430,6 +441,14 @@ def main(argv=None) -> int:
                 "frac_unpacked": round(achieved / PEAK_FP32_UNPACKED_TFLOPS, 4),
                 "kernel_ms": round(avg_kernel_s * 1e3, 4), "flops_per_fit": F,
                 "hbm_gbs": round(bytes_per_fit(w.box, len(w.psf_pixels)) * n / avg_kernel_s / 1e9, 2)}
+    mix = instruction_mix(args.workload)
+    if mix:
+        # transcendental rate against the quarter-rate ceiling of the VALU lanes
+        trans = mix["valu_trans_f32"] * n / avg_kernel_s
+        roofline["trans_per_fit"] = mix["valu_trans_f32"]
+        roofline["trans_util"] = round(trans / (VALU_LANE_OPS_PER_S / 4), 4)
+        roofline["valu_insts_per_fit"] = mix["valu"]
+        roofline["valu_issue_util"] = round(mix["valu"] * n / avg_kernel_s / VALU_LANE_OPS_PER_S, 4)
 
     cpu = None
     if not args.no_cpu_baseline:
