@@ -6,6 +6,7 @@ GaussMLEFitter / fit API (src/interface.jl) and its result conversion
 (src/roi_batch.jl).
 """
 from . import _lib, synth
+from .boxer import find_boxes, getboxes
 from ._lib import ArgumentError, GaussMLEError
 from .fitter import (CPU, GPU, FitArrays, GaussMLEFitter, auto_device, fit, fit_arrays, pvalues, select_device,
                      validate_fit_input)
@@ -23,5 +24,5 @@ __all__ = [
     "PSFModel", "ParameterConstraints", "default_constraints", "to_pixel_units", "IdealCamera", "SCMOSCamera",
     "ROIBatch", "to_electrons", "variance_map", "BasicSMLD", "LocalizationResult", "to_emitter", "to_smld",
     "Emitter2DFitGaussMLE", "Emitter2DFitSigma", "Emitter2DFitSigmaXY", "Emitter3DFitGaussMLE", "ArgumentError",
-    "GaussMLEError", "synth",
+    "GaussMLEError", "synth", "getboxes", "find_boxes",
 ]

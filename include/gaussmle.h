@@ -206,6 +206,41 @@ int gmle_epilogue_device(int64_t n_rois, int model, int box, const int8_t* d_mod
 float gmle_chi2_pvalue(float chi2, int df);
 
 /* ---------------------------------------------------------------------------
+ * ROI extraction on the device: the step before the fit, SMLMBoxer.getboxes
+ * in GaussMLE.jl's pipeline (examples/full_pipeline_boxer.jl:70-80,
+ * examples/full_pipeline.jl:73-81; SMLMBoxer is a separate package, listed
+ * unversioned in examples/Project.toml:9).  Frames are Float32 (ny, nx,
+ * n_frames), column-major (row index fastest), in photons or electrons.
+ *   D = G(sigma_small) * frame - G(sigma_large) * frame, separable, replicated
+ *       edges, 1-D weights exp(-k^2 / 2 sigma^2) for |k| <= floor(3 sigma +
+ *       0.5) normalised to sum 1 (gmle_boxer_weights), columns then rows;
+ *   a detection is a pixel with D > minval that no pixel within Chebyshev
+ *       distance ceil(boxsize - overlap) - 1 (at least 1) beats (larger D, or
+ *       equal D and earlier in column-major order): kept detections are at
+ *       least boxsize - overlap pixels apart;
+ *   its box's top-left corner is the detection minus (boxsize - 1) / 2,
+ *       clamped into the frame; ROIs hold the raw frame pixels.
+ * Detections are ordered by frame, then column-major pixel index.  Outputs
+ * (capacity max_boxes): rois (boxsize, boxsize, n) column-major, 1-based
+ * x (column) and y (row) corners and frame indices (a ROIBatch), and the
+ * optional strength D at each detection.  The count of all detections goes
+ * to *d_count (device) / *n_found (host); only the first max_boxes are
+ * written.  boxsize in [3, 16], overlap in [0, boxsize), sigmas up to 4 px.
+ * ------------------------------------------------------------------------- */
+int64_t gmle_boxer_workspace_bytes(int ny, int nx, int n_frames);
+int gmle_find_boxes_device(const float* d_frames, int ny, int nx, int n_frames, int boxsize, float overlap,
+                           float sigma_small, float sigma_large, float minval, int64_t max_boxes, float* d_rois,
+                           int32_t* d_x_corners, int32_t* d_y_corners, int32_t* d_frame_indices,
+                           float* d_strength, int64_t* d_count, void* d_workspace, void* stream);
+int gmle_find_boxes(const float* frames, int ny, int nx, int n_frames, int boxsize, float overlap,
+                    float sigma_small, float sigma_large, float minval, int64_t max_boxes, float* rois,
+                    int32_t* x_corners, int32_t* y_corners, int32_t* frame_indices, float* strength,
+                    int64_t* n_found);
+/* The filter's half-kernel w[0..R] for one sigma; returns R (-1 if sigma is
+ * not positive, R > 12, or capacity < R + 1). */
+int gmle_boxer_weights(float sigma, float* w, int capacity);
+
+/* ---------------------------------------------------------------------------
  * Serving: a resident fitting service with continuous batching.
  *
  * The reference is a library call per batch: upload, launch, wait, download
