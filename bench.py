@@ -151,6 +151,85 @@ def run_mixed(n: int, steps: int, warmup: int, precision: str, tune_reps: int = 
     return mb, res, dev_b
 
 
+BOXER = {"ny": 2048, "nx": 2048, "boxsize": 11, "overlap": 2.0, "sigma_small": 1.0, "sigma_large": 2.0,
+         "minval": 5.0, "spots_per_frame": 1000}
+
+
+def run_boxer(n_frames: int = 32, reps: int = 10, cpu: bool = True) -> dict:
+    """ROI extraction on 2048 x 2048 frames (one call over the whole stack),
+    then the pipeline frames -> boxes -> fits (XYNB 11 x 11, 20 iterations)
+    with the detection count read back between the two."""
+    import torch
+    from gaussmle_amd import boxer as gbx
+    from gaussmle_amd import synth
+    from gaussmle_amd import device as gd
+    from gaussmle_amd.models import GaussianXYNB, default_constraints
+    c = BOXER
+    ny, nx = c["ny"], c["nx"]
+    frames, _ = synth.spot_frames(ny, nx, n_frames, c["spots_per_frame"], seed=5)
+    d = torch.from_numpy(frames.reshape(-1, order="F")).cuda()
+    cap = 2 * n_frames * c["spots_per_frame"]
+    args = (ny, nx, n_frames, c["boxsize"], c["overlap"], c["sigma_small"], c["sigma_large"], c["minval"])
+    out = gbx.find_boxes_device(d, *args, max_boxes=cap)
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(s)
+    for _ in range(reps):
+        gbx.find_boxes_device(d, *args, max_boxes=cap, out=out)
+    b.record(s)
+    b.synchronize()
+    t = a.elapsed_time(b) / reps / 1e3
+    n = int(out["count"].item())
+    npix = ny * nx * n_frames
+    win = gbx.suppression_radius(c["boxsize"], c["overlap"])
+    # algorithmic HBM bytes: frames read, filtered stack written, candidate /
+    # survivor words (8 B per 64 pixels, written and read twice), each
+    # detection's window re-read and its box read and written
+    per_pix = 4 + 4 + 4 * 8 / 64
+    per_det = (2 * win + 1) ** 2 * 4 + 2 * c["boxsize"] ** 2 * 4 + 16
+    algo = per_pix * npix + per_det * n
+    res = {"frames": n_frames, "frame_shape": [ny, nx], "detections": n, "ms_per_call": round(t * 1e3, 4),
+           "frames_per_s": round(n_frames / t, 1), "pixels_per_s": round(npix / t, 1),
+           "algorithmic_bytes_per_pixel": round(algo / npix, 3), "hbm_gbs": round(algo / t / 1e9, 1),
+           "frac_hbm": round(algo / t / 1e9 / PEAK_HBM_GBS, 4),
+           "params": {k: v for k, v in c.items() if k not in ("ny", "nx")},
+           "data": "synthetic: seeded pixel-integrated Gaussian spots, flat background, Poisson-limit noise"}
+
+    # frames -> boxes -> fits
+    psf = GaussianXYNB(1.3)
+    cons = default_constraints(psf, 11)
+    fit_out = None
+
+    def pipeline():
+        nonlocal fit_out
+        gbx.find_boxes_device(d, *args, max_boxes=cap, out=out)
+        m = min(int(out["count"].item()), cap)
+        fit_out = gd.fit_device(psf, 11, out["rois"][:m].reshape(-1), cons, 20)
+        return m
+
+    pipeline()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        m = pipeline()
+    torch.cuda.synchronize()
+    tp = (time.perf_counter() - t0) / reps
+    res["pipeline"] = {"ms_per_call": round(tp * 1e3, 4), "frames_per_s": round(n_frames / tp, 1),
+                       "fits_per_s": round(m / tp, 1),
+                       "note": "find boxes, read the count back, fit every box; wall clock"}
+    if cpu:
+        from oracle import refboxer
+        f1 = np.asfortranarray(frames[:, :, :2])
+        refboxer.find(f1[:, :, :1], c["boxsize"], c["overlap"], c["sigma_small"], c["sigma_large"], c["minval"])
+        t0 = time.perf_counter()
+        refboxer.find(f1, c["boxsize"], c["overlap"], c["sigma_small"], c["sigma_large"], c["minval"])
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": round(2 / dt, 2), "unit": "frames/s", "cores": 1, "kind": "port",
+                               "sample": f"2 frames of the same stack, oracle/refboxer.cpp, {dt:.2f}s"}
+    return res
+
+
 def run_service(n_frames: int = 300, submissions: int = 3000, loads=(0.1, 0.5, 0.9, 1.0), slots: int = 4,
                 lanes=None, seed: int = 4):
     """Streaming service: seeded frames (ROI count log-uniform 50-5000, box
@@ -344,13 +423,15 @@ def main(argv=None) -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="xynb11", choices=sorted(WORKLOADS) + ["mixed", "serve"])
+    ap.add_argument("--workload", default="xynb11", choices=sorted(WORKLOADS) + ["mixed", "serve", "boxer"])
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-lanes", type=int, default=2)
     ap.add_argument("--mixed-rois", type=int, default=1_000_000,
                     help="ROIs of the uneven batch reported beside the headline line (0: skip)")
     ap.add_argument("--serve-frames", type=int, default=300,
                     help="frames of the streaming-service run reported beside the headline line (0: skip)")
+    ap.add_argument("--boxer-frames", type=int, default=32,
+                    help="2048x2048 frames of the ROI-extraction run reported beside the headline line (0: skip)")
     ap.add_argument("--rois", type=int, default=0, help="ROIs per step (default: workload's)")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "mixed"])
     ap.add_argument("--lanes", type=int, default=0)
@@ -373,6 +454,10 @@ def main(argv=None) -> int:
 
     if args.workload == "mixed":
         return main_mixed(args)
+    if args.workload == "boxer":
+        print(json.dumps({"metric": METRIC, "roi_extraction": run_boxer(args.boxer_frames or 32, args.steps,
+                                                                       cpu=not args.no_cpu_baseline)}), flush=True)
+        return 0
     if args.workload == "serve":
         res = run_service(args.serve_frames or 300, slots=args.serve_slots, lanes=args.serve_lanes)
         print(json.dumps({"metric": METRIC, "serving": res}), flush=True)
@@ -471,6 +556,9 @@ def main(argv=None) -> int:
     serving = None
     if args.workload == "xynb11" and args.serve_frames > 0:
         serving = run_service(args.serve_frames)
+    roi_extraction = None
+    if args.workload == "xynb11" and args.boxer_frames > 0:
+        roi_extraction = run_boxer(args.boxer_frames, cpu=not args.no_cpu_baseline)
 
     rec = {
         "metric": METRIC,
@@ -494,6 +582,7 @@ def main(argv=None) -> int:
         "epilogue": epilogue,
         "batched": batched,
         "serving": serving,
+        "roi_extraction": roi_extraction,
     }
     print(json.dumps(rec), flush=True)
     return 0

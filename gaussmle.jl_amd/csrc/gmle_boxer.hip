@@ -21,11 +21,14 @@
 //   order: by frame, then column-major pixel index (Julia's findall order).
 //
 // Kernels (all deterministic; no atomics):
-//   1. boxer_dog: one 64-row x 32-column tile per block, staged through LDS
-//      with a halo of R + 1; writes D and, per tile column, one 64-bit ballot
-//      word of 3x3 maxima above minval (a necessary condition).
-//   2. boxer_suppress: one thread per candidate word; the full window test on
-//      D for each set bit; per-block survivor counts.
+//   1. boxer_dog: one tile of 62 rows x 30 columns per block, staged through
+//      LDS with a halo of R + 1; each pass runs over register strips of 8
+//      outputs (one LDS read per input value and strip); writes D and, per
+//      tile column, a ballot word of the 3x3 maxima above minval (a necessary
+//      condition).
+//   2. boxer_suppress: one candidate word per lane; a wave tests its lanes'
+//      candidates one at a time, reading each window with all 64 lanes;
+//      per-block survivor counts.
 //   3. boxer_scan: exclusive scan of the block counts; total detections.
 //   4. boxer_place: each survivor's rank from the scan -> corners and frame.
 //   5. boxer_cut: one wave per box copies its pixels.
@@ -42,11 +45,14 @@
 namespace gmle {
 namespace {
 
-constexpr int kTileRows = 64;      // rows per tile (the contiguous axis): one ballot word
-constexpr int kTileCols = 32;      // columns per tile
+constexpr int kDRows = 64;         // filtered region per tile: rows (the contiguous axis) ...
+constexpr int kDCols = 32;         // ... and columns, with a 1-pixel halo around the interior
+constexpr int kWordRows = kDRows - 2;   // 62 interior rows: one candidate word per tile column
+constexpr int kInnerCols = kDCols - 2;  // 30 interior columns
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
 constexpr int kMaxRadius = 12;     // weights radius limit: sigma <= 4 px
+constexpr int kStrip = 8;          // outputs per thread along the filter axis
 constexpr int kScanThreads = 1024;
 constexpr int kCutBlocks = 4096;
 
@@ -54,17 +60,17 @@ struct BoxerArgs {
   const float* frames;   // (ny, nx, nf) column-major
   int ny, nx, nf;
   int rs, rl;            // weight radii of the two kernels
-  int halo;              // max(rs, rl): the tile's halo is halo + 1
+  int halo;              // max(rs, rl)
   float ws[kMaxRadius + 1];
   float wl[kMaxRadius + 1];
   float minval;
   int win;               // suppression radius r
   int box;
-  int wpc;               // candidate words per column: ceil(ny / 64)
+  int wpc;               // candidate words per column: ceil(ny / 62)
   int64_t n_words;       // nf * nx * wpc
   int64_t max_boxes;
   float* dog;            // (ny, nx, nf)
-  uint64_t* cand;        // (wpc, nx, nf)
+  uint64_t* cand;        // (wpc, nx, nf); bit b of word w <-> row 62 w + b
   uint64_t* keep;        // (wpc, nx, nf)
   int64_t* block_base;   // per suppress / place block
   int64_t* count;        // total detections
@@ -75,73 +81,134 @@ struct BoxerArgs {
   float* strength;       // D at the detection (may be null)
 };
 
-__device__ __forceinline__ float filter_pass(const float* p, int step, const float* w, int R) {
+// LDS layout of a tile for radii up to RMAX.  Rows and columns are stored
+// with an offset of RMAX, so every strip reads the same 8 + 2 RMAX values
+// whatever the actual radius (the values beyond it are never used), and the
+// row stride is odd, so lanes reading one value per column hit different banks.
+template <int RMAX>
+struct TileLayout {
+  static constexpr int kRows = kDRows + 2 * RMAX;         // stored rows of `in`, `hs`, `hl`
+  static constexpr int kStride = kRows | 1;
+  static constexpr int kInCols = kDCols + 2 * RMAX;
+  static constexpr int kDStride = kDRows | 1;
+  static constexpr int kWeights = 2 * (kMaxRadius + 1);
+  static constexpr int kFloats = kWeights + kInCols * kStride + 2 * kDCols * kStride;
+  static constexpr int kStripLen = kStrip + 2 * RMAX;
+};
+
+// One Gaussian over a register strip: out[o] = w0 x[c+o] + sum_k w_k
+// (x[c+o-k] + x[c+o+k]), fma in increasing k, c = RMAX.  R is uniform.
+template <int RMAX>
+__device__ __forceinline__ void strip_filter(const float (&x)[kStrip + 2 * RMAX], const float* w, int R,
+                                             float (&out)[kStrip]) {
 #pragma clang fp contract(off)
-  float v = w[0] * p[0];
-  for (int k = 1; k <= R; ++k) v = fmaf(w[k], p[-k * step] + p[k * step], v);
-  return v;
+  const float w0 = w[0];
+#pragma unroll
+  for (int o = 0; o < kStrip; ++o) out[o] = w0 * x[RMAX + o];
+#pragma unroll
+  for (int k = 1; k <= RMAX; ++k) {
+    if (k <= R) {
+      const float wk = w[k];
+#pragma unroll
+      for (int o = 0; o < kStrip; ++o) out[o] = fmaf(wk, x[RMAX + o - k] + x[RMAX + o + k], out[o]);
+    }
+  }
 }
 
+template <int RMAX>
+__device__ __forceinline__ void strip_load(const float* src, int step, float (&x)[kStrip + 2 * RMAX]) {
+#pragma unroll
+  for (int m = 0; m < kStrip + 2 * RMAX; ++m) x[m] = src[m * step];
+}
+
+// Tile (blockIdx.x, blockIdx.y) of frame blockIdx.z: interior rows
+// [62 by, 62 by + 62), columns [30 bx, 30 bx + 30); the filtered region adds
+// one pixel around it, the input region R more.
+template <int RMAX>
 __global__ __launch_bounds__(kBlock) void boxer_dog(BoxerArgs a) {
 #pragma clang fp contract(off)
-  extern __shared__ float lds[];
-  const int R = a.halo, H = R + 1;
+  using T = TileLayout<RMAX>;
+  __shared__ float lds[T::kFloats];
+  const int R = a.halo;
   const int f = blockIdx.z;
-  const int c0 = blockIdx.x * kTileCols, r0 = blockIdx.y * kTileRows;
-  const int IR = kTileRows + 2 * H, IC = kTileCols + 2 * H;   // input region, column-major [IC][IR]
-  const int HR = IR, HC = kTileCols + 2;                       // column-pass output [HC][HR]
-  const int DR = kTileRows + 2;                                // D with a 1-pixel halo [HC][DR]
-  float* in = lds;
-  float* hs = in + IC * IR;
-  float* hl = hs + HC * HR;
-  float* d = in;                                               // reuses `in` after the column pass
+  const int r0 = blockIdx.y * kWordRows - 1, c0 = blockIdx.x * kInnerCols - 1;   // filtered region origin
+  constexpr int S = T::kStride;
+  float* wgt = lds;
+  float* in = wgt + T::kWeights;                 // [kInCols][S]: column c, row r <-> (c0 + c - RMAX, r0 + r - RMAX)
+  float* hs = in + T::kInCols * S;               // [kDCols][S]: column c, row r <-> (c0 + c, r0 + r - RMAX)
+  float* hl = hs + kDCols * S;
+  float* d = in;                                 // [kDCols][kDStride], reuses `in` after the column pass
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (threadIdx.x <= kMaxRadius) {
+    wgt[threadIdx.x] = a.ws[threadIdx.x];
+    wgt[kMaxRadius + 1 + threadIdx.x] = a.wl[threadIdx.x];
+  }
+  const float* ws = wgt;
+  const float* wl = wgt + kMaxRadius + 1;
   const size_t plane = (size_t)a.ny * a.nx;
   const float* src = a.frames + (size_t)f * plane;
-  for (int e = threadIdx.x; e < IC * IR; e += kBlock) {
-    const int c = e / IR, r = e - c * IR;
-    const int gc = min(max(c0 - H + c, 0), a.nx - 1);
-    const int gr = min(max(r0 - H + r, 0), a.ny - 1);
-    in[e] = src[(size_t)gc * a.ny + gr];
+  const int lo = RMAX - R, hi_r = RMAX + kDRows + R, hi_c = RMAX + kDCols + R;
+  // stage the input region with replicated edges (lanes along the contiguous rows)
+  for (int c = lo + wave; c < hi_c; c += kWaves) {
+    const float* col = src + (size_t)min(max(c0 + c - RMAX, 0), a.nx - 1) * a.ny;
+    for (int r = lo + lane; r < hi_r; r += 64) in[c * S + r] = col[min(max(r0 + r - RMAX, 0), a.ny - 1)];
   }
   __syncthreads();
-  // along x: column cd of the D region is input column cd + R
-  for (int e = threadIdx.x; e < HC * HR; e += kBlock) {
-    const int c = e / HR, r = e - c * HR;
-    const float* p = in + (c + R) * IR + r;
-    hs[e] = filter_pass(p, IR, a.ws, a.rs);
-    hl[e] = filter_pass(p, IR, a.wl, a.rl);
+  // along x: each wave takes 8 of the 32 columns, each lane one row
+  {
+    const int cb = wave * kStrip;
+    for (int r = lo + lane; r < hi_r; r += 64) {
+      float x[T::kStripLen], vs[kStrip], vl[kStrip];
+      strip_load<RMAX>(in + cb * S + r, S, x);
+      strip_filter<RMAX>(x, ws, a.rs, vs);
+      strip_filter<RMAX>(x, wl, a.rl, vl);
+#pragma unroll
+      for (int o = 0; o < kStrip; ++o) {
+        hs[(cb + o) * S + r] = vs[o];
+        hl[(cb + o) * S + r] = vl[o];
+      }
+    }
   }
   __syncthreads();
-  // along y: row rd of the D region is row rd + R of the column-pass output
-  for (int e = threadIdx.x; e < HC * DR; e += kBlock) {
-    const int c = e / DR, r = e - c * DR;
-    const int o = c * HR + r + R;
-    d[e] = filter_pass(hs + o, 1, a.ws, a.rs) - filter_pass(hl + o, 1, a.wl, a.rl);
+  // along y: each thread takes 8 rows of one column (lanes across columns)
+  {
+    const int c = lane & 31, rb = (wave * 2 + (lane >> 5)) * kStrip;
+    float x[T::kStripLen], vs[kStrip], vl[kStrip];
+    strip_load<RMAX>(hs + c * S + rb, 1, x);
+    strip_filter<RMAX>(x, ws, a.rs, vs);
+    strip_load<RMAX>(hl + c * S + rb, 1, x);
+    strip_filter<RMAX>(x, wl, a.rl, vl);
+    // `d` overwrites `in`, which nothing reads after the column pass
+#pragma unroll
+    for (int o = 0; o < kStrip; ++o) d[c * T::kDStride + rb + o] = vs[o] - vl[o];
   }
   __syncthreads();
-  // one wave per tile column, one lane per row
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // one wave per interior column, one lane per region row (rows 1..62 interior)
+  constexpr int DS = T::kDStride;
   float* dst = a.dog + (size_t)f * plane;
-  for (int cc = wave; cc < kTileCols; cc += kWaves) {
+  for (int cc = 1 + wave; cc <= kInnerCols; cc += kWaves) {
     const int gc = c0 + cc, gr = r0 + lane;
+    const bool interior = lane >= 1 && lane <= kWordRows && gc < a.nx && gr < a.ny;
     bool cand = false;
-    if (gc < a.nx && gr < a.ny) {
-      const float v = d[(cc + 1) * DR + lane + 1];
+    if (interior) {
+      const float v = d[cc * DS + lane];
       dst[(size_t)gc * a.ny + gr] = v;
       if (v > a.minval) {
         cand = true;
+#pragma unroll
         for (int dc = -1; dc <= 1; ++dc) {
           if (gc + dc < 0 || gc + dc >= a.nx) continue;
+#pragma unroll
           for (int dr = -1; dr <= 1; ++dr) {
             if (gr + dr < 0 || gr + dr >= a.ny || (dc == 0 && dr == 0)) continue;
-            const float q = d[(cc + 1 + dc) * DR + lane + 1 + dr];
+            const float q = d[(cc + dc) * DS + lane + dr];
             const bool first = dc < 0 || (dc == 0 && dr < 0);
             if (q > v || (q == v && first)) cand = false;
           }
         }
       }
     }
-    const uint64_t word = __ballot(cand);
+    const uint64_t word = __ballot(cand) >> 1;
     if (lane == 0 && gc < a.nx) a.cand[((size_t)f * a.nx + gc) * a.wpc + blockIdx.y] = word;
   }
 }
@@ -164,38 +231,44 @@ __device__ int64_t block_exclusive_scan(int64_t v, int64_t* sh, int64_t* total) 
   return incl - v;
 }
 
+// One word per lane; the wave tests its lanes' candidates one at a time,
+// all 64 lanes reading the candidate's window together.
 __global__ __launch_bounds__(kBlock) void boxer_suppress(BoxerArgs a) {
   __shared__ int64_t sh[kBlock];
+  const int lane = threadIdx.x & 63;
   const int64_t w = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  uint64_t keep = 0;
-  if (w < a.n_words) {
-    uint64_t m = a.cand[w];
-    if (m) {
-      const int wr = (int)(w % a.wpc);
-      const int64_t fc = w / a.wpc;
-      const int gc = (int)(fc % a.nx), f = (int)(fc / a.nx);
-      const float* D = a.dog + (size_t)f * a.ny * a.nx;
-      while (m) {
-        const int b = __builtin_ctzll(m);
-        m &= m - 1;
-        const int gr = wr * 64 + b;
-        const float v = D[(size_t)gc * a.ny + gr];
-        const int cl = max(gc - a.win, 0), ch = min(gc + a.win, a.nx - 1);
-        const int rlo = max(gr - a.win, 0), rhi = min(gr + a.win, a.ny - 1);
-        bool ok = true;
-        for (int qc = cl; qc <= ch && ok; ++qc) {
-          const float* col = D + (size_t)qc * a.ny;
-          for (int qr = rlo; qr <= rhi; ++qr) {
-            const float q = col[qr];
-            const bool first = qc < gc || (qc == gc && qr < gr);
-            if (q > v || (q == v && first)) { ok = false; break; }
-          }
-        }
-        if (ok) keep |= 1ull << b;
-      }
+  const uint64_t m0 = w < a.n_words ? a.cand[w] : 0;
+  uint64_t todo = m0, keep = m0;
+  const int side = 2 * a.win + 1, area = side * side;
+  for (;;) {
+    const uint64_t busy = __ballot(todo != 0);
+    if (!busy) break;
+    const int src = __builtin_ctzll(busy);
+    // the candidate: lowest pending bit of lane `src`
+    const uint64_t ms = __shfl(todo, src);
+    const int64_t ws = __shfl(w, src);
+    const int b = __builtin_ctzll(ms);
+    const int wr = (int)(ws % a.wpc);
+    const int64_t fc = ws / a.wpc;
+    const int gc = (int)(fc % a.nx), f = (int)(fc / a.nx);
+    const int gr = wr * kWordRows + b;
+    const float* D = a.dog + (size_t)f * a.ny * a.nx;
+    const float v = D[(size_t)gc * a.ny + gr];
+    bool beaten = false;
+    for (int e = lane; e < area; e += 64) {
+      const int qc = gc - a.win + e / side, qr = gr - a.win + e % side;
+      if (qc < 0 || qc >= a.nx || qr < 0 || qr >= a.ny) continue;
+      const float q = D[(size_t)qc * a.ny + qr];
+      const bool first = qc < gc || (qc == gc && qr < gr);
+      beaten |= q > v || (q == v && first);
     }
-    a.keep[w] = keep;
+    const bool lost = __ballot(beaten) != 0;
+    if (lane == src) {
+      todo &= todo - 1;
+      if (lost) keep &= ~(1ull << b);
+    }
   }
+  if (w < a.n_words) a.keep[w] = keep;
   int64_t total;
   block_exclusive_scan(__popcll(keep), sh, &total);
   if (threadIdx.x == 0) a.block_base[blockIdx.x] = total;
@@ -235,7 +308,7 @@ __global__ __launch_bounds__(kBlock) void boxer_place(BoxerArgs a) {
   while (keep && k < a.max_boxes) {
     const int b = __builtin_ctzll(keep);
     keep &= keep - 1;
-    const int gr = wr * 64 + b;
+    const int gr = wr * kWordRows + b;
     const int r0 = min(max(gr - half, 0), a.ny - a.box);
     a.xc[k] = c0 + 1;
     a.yc[k] = r0 + 1;
@@ -283,7 +356,7 @@ struct Layout {
 
 Layout layout(int ny, int nx, int nf) {
   Layout L;
-  L.wpc = (ny + kTileRows - 1) / kTileRows;
+  L.wpc = (ny + kWordRows - 1) / kWordRows;
   L.n_words = (int64_t)nf * nx * L.wpc;
   L.n_blocks = (L.n_words + kBlock - 1) / kBlock;
   L.dog = 0;
@@ -334,16 +407,14 @@ BoxerArgs make_args(const float* frames, int ny, int nx, int nf, int box, float 
   return a;
 }
 
-size_t dog_lds_bytes(int halo_radius) {
-  const int H = halo_radius + 1;
-  const int IR = kTileRows + 2 * H, IC = kTileCols + 2 * H, HC = kTileCols + 2;
-  return (size_t)(IC * IR + 2 * HC * IR) * sizeof(float);
-}
 
 int enqueue_boxer(BoxerArgs a, hipStream_t stream) {
   const Layout L = layout(a.ny, a.nx, a.nf);
-  const dim3 grid((a.nx + kTileCols - 1) / kTileCols, L.wpc, a.nf);
-  hipLaunchKernelGGL(boxer_dog, grid, dim3(kBlock), dog_lds_bytes(a.halo), stream, a);
+  const dim3 grid((a.nx + kInnerCols - 1) / kInnerCols, L.wpc, a.nf);
+  if (a.halo <= 6)
+    hipLaunchKernelGGL(boxer_dog<6>, grid, dim3(kBlock), 0, stream, a);
+  else
+    hipLaunchKernelGGL(boxer_dog<kMaxRadius>, grid, dim3(kBlock), 0, stream, a);
   hipLaunchKernelGGL(boxer_suppress, dim3((unsigned)L.n_blocks), dim3(kBlock), 0, stream, a);
   hipLaunchKernelGGL(boxer_scan, dim3(1), dim3(kScanThreads), 0, stream, a.block_base, L.n_blocks, a.count);
   hipLaunchKernelGGL(boxer_place, dim3((unsigned)L.n_blocks), dim3(kBlock), 0, stream, a);

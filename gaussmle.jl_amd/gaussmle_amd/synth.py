@@ -211,3 +211,33 @@ def generic(psf_pixels: PSFModel, n: int, box: int, seed: int, iterations: int =
     yc = rng.integers(1, cam - box + 2, n).astype(np.int32)
     data = scmos_stack(rng, mu, var, gain, offset, xc, yc)
     return Workload(name, psf_pixels, box, data, theta, constraints, iterations, var, xc, yc)
+
+
+def spot_frames(ny: int, nx: int, nf: int, spots_per_frame: int, seed: int = 5, bg: float = 10.0,
+                photons=(500.0, 3000.0), sigma: float = 1.3, edge: int = 4, window: int = 8):
+    """Camera frames (ny, nx, nf) Float32, column-major, in photons: a flat
+    background plus pixel-integrated Gaussian spots (each added within +-window
+    pixels of its centre), with Gaussian noise of variance equal to the mean
+    (the Poisson limit).  Returns (frames, truth) with truth rows (frame, y, x,
+    photons), 0-based pixel coordinates."""
+    rng = np.random.default_rng(seed)
+    frames = np.empty((ny, nx, nf), dtype=F32, order="F")
+    truth = []
+    k = np.arange(-window, window + 2, dtype=np.float64) - 0.5
+    for f in range(nf):
+        mu = np.full((ny, nx), bg, dtype=np.float64)
+        ys = rng.uniform(edge, ny - 1 - edge, spots_per_frame)
+        xs = rng.uniform(edge, nx - 1 - edge, spots_per_frame)
+        Ns = rng.uniform(*photons, spots_per_frame)
+        for y, x, N in zip(ys, xs, Ns):
+            iy, ix = int(round(y)), int(round(x))
+            py = np.diff(0.5 * erf((iy + k - y) / (np.sqrt(2.0) * sigma)))
+            px = np.diff(0.5 * erf((ix + k - x) / (np.sqrt(2.0) * sigma)))
+            r0, r1 = max(iy - window, 0), min(iy + window + 1, ny)
+            c0, c1 = max(ix - window, 0), min(ix + window + 1, nx)
+            mu[r0:r1, c0:c1] += N * np.outer(py[r0 - iy + window:r1 - iy + window],
+                                             px[c0 - ix + window:c1 - ix + window])
+            truth.append((f, y, x, N))
+        noise = rng.standard_normal((ny, nx), dtype=np.float32)
+        frames[:, :, f] = (mu + np.sqrt(mu) * noise).astype(F32)
+    return frames, np.array(truth)
