@@ -147,17 +147,34 @@ __global__ __launch_bounds__(kBlock) void boxer_dog(BoxerArgs a) {
   const float* wl = wgt + kMaxRadius + 1;
   const size_t plane = (size_t)a.ny * a.nx;
   const float* src = a.frames + (size_t)f * plane;
-  const int lo = RMAX - R, hi_r = RMAX + kDRows + R, hi_c = RMAX + kDCols + R;
-  // stage the input region with replicated edges (lanes along the contiguous rows)
-  for (int c = lo + wave; c < hi_c; c += kWaves) {
-    const float* col = src + (size_t)min(max(c0 + c - RMAX, 0), a.nx - 1) * a.ny;
-    for (int r = lo + lane; r < hi_r; r += 64) in[c * S + r] = col[min(max(r0 + r - RMAX, 0), a.ny - 1)];
+  const int lo = RMAX - R, hi_r = RMAX + kDRows + R;
+  // stage the input region (radius RMAX) with replicated edges: every load of
+  // a thread is issued before the first LDS write, so they overlap in flight
+  {
+    constexpr int kIn = T::kInCols * T::kRows;
+    constexpr int kPer = (kIn + kBlock - 1) / kBlock;
+    float v[kPer];
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+      const int e = threadIdx.x + i * kBlock;
+      const int c = e / T::kRows, r = e - c * T::kRows;
+      if (i + 1 < kPer || e < kIn)
+        v[i] = src[(size_t)min(max(c0 + c - RMAX, 0), a.nx - 1) * a.ny + min(max(r0 + r - RMAX, 0), a.ny - 1)];
+    }
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+      const int e = threadIdx.x + i * kBlock;
+      const int c = e / T::kRows, r = e - c * T::kRows;
+      if (i + 1 < kPer || e < kIn) in[c * S + r] = v[i];
+    }
   }
   __syncthreads();
-  // along x: each wave takes 8 of the 32 columns, each lane one row
+  // along x, over rows [lo, hi_r) and 4 strips of 8 columns: first rows
+  // lo..lo+63 (each wave one strip, each lane one row), then the 2R remaining
+  // rows x 4 strips packed onto the fewest lanes (threads 0 .. 8R-1), so the
+  // other waves leave their issue slots to other blocks
   {
-    const int cb = wave * kStrip;
-    for (int r = lo + lane; r < hi_r; r += 64) {
+    auto x_pass = [&](int r, int cb) {
       float x[T::kStripLen], vs[kStrip], vl[kStrip];
       strip_load<RMAX>(in + cb * S + r, S, x);
       strip_filter<RMAX>(x, ws, a.rs, vs);
@@ -167,7 +184,10 @@ __global__ __launch_bounds__(kBlock) void boxer_dog(BoxerArgs a) {
         hs[(cb + o) * S + r] = vs[o];
         hl[(cb + o) * S + r] = vl[o];
       }
-    }
+    };
+    x_pass(lo + lane, wave * kStrip);
+    const int t = threadIdx.x;
+    if (t < 4 * (hi_r - lo - 64)) x_pass(lo + 64 + (t >> 2), (t & 3) * kStrip);
   }
   __syncthreads();
   // along y: each thread takes 8 rows of one column (lanes across columns)
@@ -213,22 +233,28 @@ __global__ __launch_bounds__(kBlock) void boxer_dog(BoxerArgs a) {
   }
 }
 
-// Block-wide exclusive scan of one int64 per thread; returns the prefix and
-// leaves the block total in *total.
+// Block-wide exclusive scan of one int64 per thread (inclusive scans within
+// each wave by shuffles, then the wave totals through LDS); returns the prefix
+// and leaves the block total in *total.  sh holds at least blockDim.x / 64.
 __device__ int64_t block_exclusive_scan(int64_t v, int64_t* sh, int64_t* total) {
-  const int t = threadIdx.x;
-  sh[t] = v;
-  __syncthreads();
-  for (int off = 1; off < (int)blockDim.x; off <<= 1) {
-    const int64_t add = t >= off ? sh[t - off] : 0;
-    __syncthreads();
-    sh[t] += add;
-    __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  int64_t incl = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int64_t up = __shfl_up(incl, off);
+    if (lane >= off) incl += up;
   }
-  const int64_t incl = sh[t];
-  *total = sh[blockDim.x - 1];
+  if (lane == 63) sh[wave] = incl;
   __syncthreads();
-  return incl - v;
+  int64_t before = 0, all = 0;
+  for (int q = 0; q < nw; ++q) {
+    const int64_t t = sh[q];
+    before += q < wave ? t : 0;
+    all += t;
+  }
+  *total = all;
+  __syncthreads();
+  return before + incl - v;
 }
 
 // One word per lane; the wave tests its lanes' candidates one at a time,
@@ -240,6 +266,7 @@ __global__ __launch_bounds__(kBlock) void boxer_suppress(BoxerArgs a) {
   const uint64_t m0 = w < a.n_words ? a.cand[w] : 0;
   uint64_t todo = m0, keep = m0;
   const int side = 2 * a.win + 1, area = side * side;
+  const int step_c = 64 / side, step_r = 64 - step_c * side;
   for (;;) {
     const uint64_t busy = __ballot(todo != 0);
     if (!busy) break;
@@ -255,12 +282,21 @@ __global__ __launch_bounds__(kBlock) void boxer_suppress(BoxerArgs a) {
     const float* D = a.dog + (size_t)f * a.ny * a.nx;
     const float v = D[(size_t)gc * a.ny + gr];
     bool beaten = false;
+    // window element e = lane + 64 i <-> (column e / side, row e % side), stepped
+    int ec = lane / side, er = lane - ec * side;
     for (int e = lane; e < area; e += 64) {
-      const int qc = gc - a.win + e / side, qr = gr - a.win + e % side;
-      if (qc < 0 || qc >= a.nx || qr < 0 || qr >= a.ny) continue;
-      const float q = D[(size_t)qc * a.ny + qr];
-      const bool first = qc < gc || (qc == gc && qr < gr);
-      beaten |= q > v || (q == v && first);
+      const int qc = gc - a.win + ec, qr = gr - a.win + er;
+      if (qc >= 0 && qc < a.nx && qr >= 0 && qr < a.ny) {
+        const float q = D[(size_t)qc * a.ny + qr];
+        const bool first = qc < gc || (qc == gc && qr < gr);
+        beaten |= q > v || (q == v && first);
+      }
+      ec += step_c;
+      er += step_r;
+      if (er >= side) {
+        er -= side;
+        ++ec;
+      }
     }
     const bool lost = __ballot(beaten) != 0;
     if (lane == src) {

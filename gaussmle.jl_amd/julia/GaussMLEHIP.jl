@@ -168,6 +168,31 @@ end
 hip_service_destroy(s) = check(ccall((:gmle_service_destroy, libgmle), Cint, (Ptr{Cvoid},), s))
 
 """
+    hip_getboxes(imagestack; boxsize = 7, overlap = 2.0, sigma_small = 1.0, sigma_large = 2.0, minval = 0.0)
+
+Candidate boxes of a (rows, cols, frames) Float32 stack, as SMLMBoxer.getboxes
+cuts them (examples/full_pipeline.jl:73-81): returns (rois, x_corners,
+y_corners, frames) with 1-based corners, ordered by frame, then column-major
+pixel index.
+"""
+function hip_getboxes(imagestack::Array{Float32,3}; boxsize::Integer = 7, overlap::Real = 2.0,
+                      sigma_small::Real = 1.0, sigma_large::Real = 2.0, minval::Real = 0.0)
+    ny, nx, nf = size(imagestack)
+    r = max(1, ceil(Int, Float32(boxsize) - Float32(overlap)) - 1) + 1
+    cap = nf * cld(ny, r) * cld(nx, r)   # at most one detection per r x r cell
+    rois = Array{Float32,3}(undef, boxsize, boxsize, cap)
+    xc = Vector{Int32}(undef, cap); yc = Vector{Int32}(undef, cap); fr = Vector{Int32}(undef, cap)
+    n = Ref{Int64}(0)
+    check(ccall((:gmle_find_boxes, libgmle), Cint,
+                (Ptr{Cfloat}, Cint, Cint, Cint, Cint, Cfloat, Cfloat, Cfloat, Cfloat, Int64, Ptr{Cfloat},
+                 Ptr{Int32}, Ptr{Int32}, Ptr{Int32}, Ptr{Cfloat}, Ref{Int64}),
+                imagestack, ny, nx, nf, boxsize, overlap, sigma_small, sigma_large, minval, cap, rois,
+                xc, yc, fr, C_NULL, n))
+    m = n[]
+    return rois[:, :, 1:m], xc[1:m], yc[1:m], fr[1:m]
+end
+
+"""
     hip_chi2_pvalue(chi2, df)
 
 `1 - Float32(cdf(Chisq(df), chi2))` as the device epilogue computes it
