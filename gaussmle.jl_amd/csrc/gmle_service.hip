@@ -51,6 +51,10 @@ constexpr int kMaxReqPerBatch = 4096;
 // batch of its own): past it the GPU is full anyway, and splitting lets one
 // slot's PCIe copies overlap another slot's fits.
 constexpr int64_t kTargetBatchRois = 1 << 16;
+constexpr int64_t kXferBlocks = 256;
+// LDS staging of one chunk's raw counts: 16-byte vectors, plus one for a
+// start that is not 16-byte aligned.
+constexpr int kRawVecs = (kChunk * kMaxBox * kMaxBox * 2) / 16 + 1;
 // Completion polling: spin this long, then sleep between polls.
 constexpr double kSpinSeconds = 2e-3;
 // Pinned memory the GPU reads and writes directly: mapped, visible to every
@@ -118,6 +122,7 @@ constexpr int64_t kPixSlack = (int64_t)kChunk * kMaxBox * kMaxBox + kMaxBox * kM
 struct IngestArgs {
   const ReqDesc* reqs;
   const Chunk* chunks;
+  int64_t n_chunks;
   const float* offset_map;    // (ny, nx) column-major
   const float* gain_map;
   int ny;
@@ -133,6 +138,7 @@ struct IngestArgs {
 struct ResultArgs {
   const ReqDesc* reqs;
   const Chunk* chunks;
+  int64_t n_chunks;
   const float* params;
   const float* crlb;
   const float* llr;
@@ -147,9 +153,21 @@ __global__ __launch_bounds__(kIngestBlock) void service_ingest(IngestArgs a) {
   __shared__ int sxc[kChunk];
   __shared__ int syc[kChunk];
   __shared__ int spre[kChunk + 1];
-  const Chunk c = a.chunks[blockIdx.x];
-  const ReqDesc r = a.reqs[c.req];
+  __shared__ uint4 sraw[kRawVecs];
   const int t = threadIdx.x;
+  for (int64_t ci = blockIdx.x; ci < a.n_chunks; ci += gridDim.x) {
+  const Chunk c = a.chunks[ci];
+  const ReqDesc r = a.reqs[c.req];
+  // The chunk's raw counts are contiguous: fetch them with aligned 16-byte
+  // loads (over PCIe when they sit in pinned host memory, where wide requests
+  // matter) into LDS.  The aligned span never leaves the pages (host) or the
+  // arena and its slack (device) that hold the counts.
+  const int npix = min(c.npix, kChunk * kMaxBox * kMaxBox);
+  const uintptr_t beg = (uintptr_t)(r.adu + c.pix0);
+  const uintptr_t b0 = beg & ~(uintptr_t)15;
+  const int nvec = (int)((beg + 2 * (uintptr_t)npix - b0 + 15) >> 4);
+  for (int v = t; v < nvec; v += kIngestBlock) sraw[v] = ((const uint4*)b0)[v];
+  const uint16_t* raw = (const uint16_t*)sraw + ((beg - b0) >> 1);
   if (t < c.count) {
     const int k = c.first + t;
     const int64_t roi = r.roi_base + k;
@@ -178,8 +196,7 @@ __global__ __launch_bounds__(kIngestBlock) void service_ingest(IngestArgs a) {
   for (int q = wave; q < c.count; q += kIngestBlock / kWave) {
     const int b = sbox[q];
     const int p0 = spre[q];
-    if (p0 + b * b > c.npix) break;   // validated sizes never get here
-    const uint16_t* src = r.adu + c.pix0 + p0;
+    if (p0 + b * b > npix) break;   // validated sizes never get here
     float* dst = a.data + r.pix_base + c.pix0 + p0;
     const int64_t m0 = (int64_t)(syc[q] - 1) + (int64_t)a.ny * (int64_t)(sxc[q] - 1);
 #pragma unroll
@@ -189,24 +206,51 @@ __global__ __launch_bounds__(kIngestBlock) void service_ingest(IngestArgs a) {
         const int i = p % b, j = p / b;   // row, column inside the ROI
         int64_t m = m0 + i + (int64_t)a.ny * j;
         m = m < 0 ? 0 : (m >= a.map_size ? a.map_size - 1 : m);   // corners are validated; never read outside
-        dst[p] = ((float)src[p] - a.offset_map[m]) * a.gain_map[m];
+        dst[p] = ((float)raw[p0 + p] - a.offset_map[m]) * a.gain_map[m];
       }
     }
   }
+  __syncthreads();   // LDS is reused by the next chunk
+  }
 }
 
-// One block per chunk: results of its ROIs into the request's output buffers.
-__global__ __launch_bounds__(kIngestBlock) void service_results(ResultArgs a) {
-  const Chunk c = a.chunks[blockIdx.x];
-  const ReqDesc r = a.reqs[c.req];
-  for (int e = threadIdx.x; e < c.count * kBatchStride; e += kIngestBlock) {
-    const int k = e / kBatchStride;
-    const int64_t src = (r.roi_base + c.first + k) * kBatchStride + e % kBatchStride;
-    const int64_t dst = (int64_t)(c.first + k) * kBatchStride + e % kBatchStride;
-    r.params[dst] = a.params[src];
-    r.crlb[dst] = a.crlb[src];
+// Copies n floats from device memory to a host-visible buffer: staged in LDS,
+// then written with 16-byte stores where the destination allows (posted
+// writes over PCIe carry 16 bytes per lane instead of 4).
+__device__ __forceinline__ void put_floats(float* dst, const float* src, int n, float* stage) {
+  const int t = threadIdx.x;
+  for (int e = t; e < n; e += kIngestBlock) stage[e] = src[e];
+  __syncthreads();
+  const int lead = (int)(((16 - ((uintptr_t)dst & 15)) & 15) >> 2);   // floats before a 16-byte boundary
+  if (((uintptr_t)dst & 3) == 0 && lead < n) {
+    const int nv = (n - lead) >> 2;
+    for (int v = t; v < nv; v += kIngestBlock) {
+      const float* s = stage + lead + 4 * v;
+      *(float4*)(dst + lead + 4 * v) = make_float4(s[0], s[1], s[2], s[3]);
+    }
+    const int tail = lead + 4 * nv;
+    if (t < lead) dst[t] = stage[t];
+    if (t >= lead && t < lead + (n - tail)) dst[tail + t - lead] = stage[tail + t - lead];
+  } else {
+    for (int e = t; e < n; e += kIngestBlock) dst[e] = stage[e];
   }
-  if (threadIdx.x < c.count) r.llr[c.first + threadIdx.x] = a.llr[r.roi_base + c.first + threadIdx.x];
+  __syncthreads();
+}
+
+// Results of each chunk's ROIs into the request's output buffers (blocks stride
+// over the chunks).
+__global__ __launch_bounds__(kIngestBlock) void service_results(ResultArgs a) {
+  __shared__ float stage[kChunk * kBatchStride];
+  for (int64_t ci = blockIdx.x; ci < a.n_chunks; ci += gridDim.x) {
+    const Chunk c = a.chunks[ci];
+    const ReqDesc r = a.reqs[c.req];
+    const int64_t s0 = r.roi_base + c.first;
+    put_floats(r.params + (int64_t)c.first * kBatchStride, a.params + s0 * kBatchStride, c.count * kBatchStride,
+               stage);
+    put_floats(r.crlb + (int64_t)c.first * kBatchStride, a.crlb + s0 * kBatchStride, c.count * kBatchStride,
+               stage);
+    put_floats(r.llr + c.first, a.llr + s0, c.count, stage);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -313,6 +357,8 @@ struct gmle_service {
   double stat_gpu_max = 0, stat_seen_max = 0;
   int launch_error = GMLE_OK;
   bool direct_ingest = false;   // ingest reads request counts from pinned memory (no DMA copies)
+  int64_t target_rois = gmle::kTargetBatchRois;   // a batch closes once it holds this many ROIs
+  int64_t xfer_blocks = gmle::kXferBlocks;        // grid of the ingest / result kernels
   std::string error_msg;        // first failure of a batch (launch or completion)
 };
 
@@ -519,9 +565,12 @@ int launch_slot(gmle_service* s, Slot& sl) {
     roi += q.n;
     pix += q.npix;
   }
-  IngestArgs ia{sl.reqs_d, sl.chunks_d, s->offset_map, s->gain_map, s->ny, (int64_t)s->ny * s->nx, sl.data, sl.models,
-                sl.boxes, sl.xc, sl.yc, sl.offsets};
-  ResultArgs ra{sl.reqs_d, sl.chunks_d, sl.params, sl.crlb, sl.llr};
+  IngestArgs ia{sl.reqs_d, sl.chunks_d, nch, s->offset_map, s->gain_map, s->ny, (int64_t)s->ny * s->nx, sl.data,
+                sl.models, sl.boxes, sl.xc, sl.yc, sl.offsets};
+  ResultArgs ra{sl.reqs_d, sl.chunks_d, nch, sl.params, sl.crlb, sl.llr};
+  // Host-memory transfers wait on PCIe: a bounded grid keeps enough requests
+  // in flight without parking every CU, so other slots' fits run beside them.
+  const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>(nch, s->xfer_blocks));
   BatchPlan bp = slot_plan(sl, roi);
   int64_t start = 0;
   for (int k = 0; k < kNumShapes; ++k) {
@@ -530,10 +579,10 @@ int launch_slot(gmle_service* s, Slot& sl) {
     start += counts[k];
   }
   const unsigned gb = (unsigned)((roi + kBatchBlock - 1) / kBatchBlock);
-  if ((e = set_kernel_node(sl.exec, sl.n_ingest, (const void*)&service_ingest, (unsigned)nch, kIngestBlock, &ia)) != hipSuccess ||
+  if ((e = set_kernel_node(sl.exec, sl.n_ingest, (const void*)&service_ingest, gx, kIngestBlock, &ia)) != hipSuccess ||
       (e = set_kernel_node(sl.exec, sl.n_init, batch_init_kernel(), gb, kBatchBlock, &bp)) != hipSuccess ||
       (e = set_kernel_node(sl.exec, sl.n_scatter, batch_scatter_kernel(), gb, kBatchBlock, &bp)) != hipSuccess ||
-      (e = set_kernel_node(sl.exec, sl.n_results, (const void*)&service_results, (unsigned)nch, kIngestBlock, &ra)) != hipSuccess)
+      (e = set_kernel_node(sl.exec, sl.n_results, (const void*)&service_results, gx, kIngestBlock, &ra)) != hipSuccess)
     return hip_fail(e, "hipGraphExecKernelNodeSetParams");
   if (sl.n_multi) {
     MultiArgs m;
@@ -611,7 +660,7 @@ void dispatcher_main(gmle_service* s) {
     while (!s->pending.empty() && (int)sl.batch.size() < kMaxReqPerBatch) {
       const Pending& q = *s->pending.front();
       if (rois + q.n > s->cap_rois || pix + q.npix > s->cap_pix) break;
-      if (rois > 0 && rois + q.n > kTargetBatchRois) break;
+      if (rois > 0 && rois + q.n > s->target_rois) break;
       rois += q.n;
       pix += q.npix;
       sl.batch.push_back(s->pending.front());
@@ -867,6 +916,16 @@ int gmle_service_create(int device, size_t hbm_budget, int slots, int precision,
   // (one copy per request into the slot before its graph).
   const char* ingest = std::getenv("GMLE_SERVICE_INGEST");
   s->direct_ingest = !(ingest && std::string(ingest) == "dma");
+  // Batch size target (ROIs): smaller batches let one slot's ingest overlap
+  // another's fit; larger ones amortize the per-batch latency floor.
+  if (const char* tb = std::getenv("GMLE_SERVICE_BATCH_ROIS")) {
+    const long long v = std::atoll(tb);
+    if (v >= 256 && v <= (1LL << 24)) s->target_rois = v;
+  }
+  if (const char* xb = std::getenv("GMLE_SERVICE_XFER_BLOCKS")) {
+    const long long v = std::atoll(xb);
+    if (v >= 1 && v <= (1LL << 20)) s->xfer_blocks = v;
+  }
   (void)hipSetDevice(device);
   s->slots.clear();
   for (int k = 0; k < slots; ++k) {
