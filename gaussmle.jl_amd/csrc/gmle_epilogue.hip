@@ -27,6 +27,11 @@
 namespace gmle {
 namespace {
 
+// Relative stopping tolerance of both expansions: a few Float64 ulp.  (A
+// tolerance below one ulp would never be met by the continued fraction, whose
+// ratio settles within an ulp of 1, and would run it to the iteration cap.)
+constexpr double kTol = 1e-15;
+
 __host__ __device__ inline double gamma_p(double a, double x) {
   if (!(x > 0.0)) return x == 0.0 ? 0.0 : __builtin_nan("");
   if (isinf(x)) return 1.0;
@@ -38,7 +43,7 @@ __host__ __device__ inline double gamma_p(double a, double x) {
       ap += 1.0;
       del *= x / ap;
       sum += del;
-      if (fabs(del) < fabs(sum) * 1e-17) break;
+      if (fabs(del) < fabs(sum) * kTol) break;
     }
     return sum * lead;
   }
@@ -54,7 +59,7 @@ __host__ __device__ inline double gamma_p(double a, double x) {
     d = 1.0 / d;
     const double del = d * c;
     h *= del;
-    if (fabs(del - 1.0) < 1e-17) break;
+    if (fabs(del - 1.0) < kTol) break;
   }
   return 1.0 - lead * h;
 }

@@ -61,6 +61,11 @@ SIGNATURES = {
     "gmle_find_boxes": (C.c_int, [_F, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
                                   C.c_float, C.c_int64, _F, _I32, _I32, _I32, _F, _I64]),
     "gmle_boxer_weights": (C.c_int, [C.c_float, _F, C.c_int]),
+    "gmle_simulate_device": (C.c_int, [C.c_int, _F, C.c_int, C.c_int64, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
+                                       C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
+                                       C.c_float, C.c_float, C.c_float, C.c_float, C.c_uint64, C.c_void_p,
+                                       C.c_void_p]),
+    "gmle_philox4x32_10": (None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "gmle_epilogue_device": (C.c_int, [C.c_int64, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
                                        C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double,
                                        C.c_void_p, C.c_void_p]),

@@ -241,6 +241,29 @@ int gmle_find_boxes(const float* frames, int ny, int nx, int n_frames, int boxsi
 int gmle_boxer_weights(float sigma, float* w, int capacity);
 
 /* ---------------------------------------------------------------------------
+ * Simulator on the device: noisy ROI stacks as GaussMLE.jl's
+ * generate_roi_batch draws them (src/simulator.jl:35-345), for tests and
+ * benchmarks.  Per pixel, Float32: expected = bg + photons * Phi_x * Phi_y;
+ * ideal camera: Poisson(expected); sCMOS: ADU = (Poisson(expected * qe) +
+ * N(0,1) * readnoise) / gain + offset, with qe / readnoise / gain / offset
+ * from (ny, nx) maps when given (indexed as the fitting kernel indexes the
+ * variance map: [y_corner + i - 1, x_corner + j - 1]) or scalars.  Random
+ * numbers: Philox4x32-10 keyed by `seed`, counted by (ROI, pixel, purpose);
+ * Poisson by multiplication (mean < 10) or PTRS (Hoermann 1993); normal by
+ * Box-Muller.  d_params holds `stride` values per ROI in the model's order;
+ * d_out is (box, box, n) in the fitting layout.  Enqueued on `stream`.
+ * Models 0-2; the astigmatic model returns GMLE_ERR_UNSUPPORTED (its device
+ * path is switched off).
+ * ------------------------------------------------------------------------- */
+int gmle_simulate_device(int model, const float* psf, int box, int64_t n_rois, const float* d_params, int stride,
+                         int camera, const int32_t* d_x_corners, const int32_t* d_y_corners, const float* d_qe,
+                         const float* d_readnoise, const float* d_gain, const float* d_offset, int ny, int nx,
+                         float qe, float readnoise, float gain, float offset, uint64_t seed, float* d_out,
+                         void* stream);
+/* The generator itself, on the host: out = Philox4x32-10(counter[4], key[2]). */
+void gmle_philox4x32_10(const uint32_t* counter, const uint32_t* key, uint32_t* out);
+
+/* ---------------------------------------------------------------------------
  * Serving: a resident fitting service with continuous batching.
  *
  * The reference is a library call per batch: upload, launch, wait, download

@@ -31,7 +31,7 @@ def build(force: bool = False) -> Path:
     """Compile the oracle with its Makefile."""
     if force or not LIB.exists() or LIB.stat().st_mtime < max(
             (HERE / "refcpu.cpp").stat().st_mtime, (HERE / "refcpu.hpp").stat().st_mtime,
-            (HERE / "refboxer.cpp").stat().st_mtime):
+            (HERE / "refboxer.cpp").stat().st_mtime, (HERE / "refsim.cpp").stat().st_mtime):
         subprocess.run(["make", "-s", "-C", str(HERE), "-B" if force else "all"], check=True)
     return LIB
 
