@@ -71,25 +71,27 @@ __host__ __device__ inline float sim_pixel(const SimArgs& a, int64_t e) {
   const int64_t k = e / BB;
   const int p = (int)(e - k * BB);
   const int i = p % B + 1, j = p / B + 1;   // row, column (1-based)
+  // The ROI's parameters as values first (the model then selects among
+  // values, not among pointers into the row).
   const float* th = a.params + k * a.stride;
-  float x = th[0], y = th[1], N, bg, sx, sy;
+  const int np = a.stride < 6 ? a.stride : 6;
+  float t[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) t[q] = q < np ? th[q] : 0.0f;
+  const float x = t[0], y = t[1];
+  const bool astig = a.model == ASTIG;
+  const float N = astig ? t[3] : t[2];
+  const float bg = astig ? t[4] : t[3];
+  float sx, sy;
   if (a.model == XYNB) {
-    N = th[2];
-    bg = th[3];
     sx = sy = a.psf[0];
   } else if (a.model == XYNBS) {
-    N = th[2];
-    bg = th[3];
-    sx = sy = th[4];
+    sx = sy = t[4];
   } else if (a.model == XYNBSXSY) {
-    N = th[2];
-    bg = th[3];
-    sx = th[4];
-    sy = th[5];
+    sx = t[4];
+    sy = t[5];
   } else {
-    const float z = th[2];
-    N = th[3];
-    bg = th[4];
+    const float z = t[2];
     sx = a.psf[0] * sqrtf(astig_alpha((z - a.psf[6]) / a.psf[7], a.psf[2], a.psf[4]));
     sy = a.psf[1] * sqrtf(astig_alpha((z + a.psf[6]) / a.psf[7], a.psf[3], a.psf[5]));
   }
@@ -146,10 +148,6 @@ int gmle_simulate_device(int model, const float* psf, int box, int64_t n_rois, c
                          void* stream) {
   int rc = check_sim(model, psf, box, n_rois, stride, camera, ny, nx);
   if (rc != GMLE_OK) return rc;
-  // The astigmatic model's device path is switched off: its one GPU run ended
-  // in a memory fault whose cause has not been found.
-  if (model == ASTIG)
-    return fail(GMLE_ERR_UNSUPPORTED, "the device simulator does not serve the astigmatic model");
   if (!d_params || !d_out) return fail(GMLE_ERR_ARG, "params and out must not be NULL");
   if (camera == SCMOS && (!d_x_corners || !d_y_corners)) return fail(GMLE_ERR_ARG, "sCMOS needs ROI corners");
   if (camera == SCMOS && (!(gain != 0.0f) && !d_gain)) return fail(GMLE_ERR_ARG, "gain must be non-zero");

@@ -252,8 +252,6 @@ int gmle_boxer_weights(float sigma, float* w, int capacity);
  * Poisson by multiplication (mean < 10) or PTRS (Hoermann 1993); normal by
  * Box-Muller.  d_params holds `stride` values per ROI in the model's order;
  * d_out is (box, box, n) in the fitting layout.  Enqueued on `stream`.
- * Models 0-2; the astigmatic model returns GMLE_ERR_UNSUPPORTED (its device
- * path is switched off).
  * ------------------------------------------------------------------------- */
 int gmle_simulate_device(int model, const float* psf, int box, int64_t n_rois, const float* d_params, int stride,
                          int camera, const int32_t* d_x_corners, const int32_t* d_y_corners, const float* d_qe,
