@@ -230,6 +230,41 @@ def run_boxer(n_frames: int = 32, reps: int = 10, cpu: bool = True) -> dict:
     return res
 
 
+def run_simulator(n: int = 1_048_576, reps: int = 10, cpu: bool = True) -> dict:
+    """The on-device simulator: n ROIs of 11 x 11 XYNB (the headline shape),
+    Poisson noise, per-ROI parameters resident on the device."""
+    import torch
+    from gaussmle_amd import simulator as gsim
+    from gaussmle_amd.models import GaussianXYNB
+    psf = GaussianXYNB(1.3)
+    rng = np.random.default_rng(9)
+    params = np.stack([rng.uniform(4.5, 7.5, n), rng.uniform(4.5, 7.5, n), rng.uniform(500, 5000, n),
+                       rng.uniform(1, 20, n)], axis=1).astype(np.float32)
+    d = torch.from_numpy(params).cuda()
+    out = gsim.simulate_device(psf, 11, d, 1)
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(s)
+    for r in range(reps):
+        gsim.simulate_device(psf, 11, d, 2 + r, out=out)
+    b.record(s)
+    b.synchronize()
+    t = a.elapsed_time(b) / reps / 1e3
+    res = {"rois": n, "box": 11, "model": "GaussianXYNB", "camera": "ideal", "ms_per_call": round(t * 1e3, 4),
+           "rois_per_s": round(n / t, 1), "pixels_per_s": round(n * 121 / t, 1),
+           "hbm_gbs": round((n * 121 * 4 + n * 16) / t / 1e9, 1)}
+    if cpu:
+        from oracle import refsim
+        m = 20_000
+        t0 = time.perf_counter()
+        refsim.simulate(psf, 11, params[:m], seed=1)
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": round(m / dt, 1), "unit": "ROIs/s", "cores": 1, "kind": "port",
+                               "sample": f"{m} ROIs, oracle/refsim.cpp, {dt:.2f}s"}
+    return res
+
+
 def run_service(n_frames: int = 300, submissions: int = 3000, loads=(0.1, 0.5, 0.9, 1.0), slots: int = 4,
                 lanes=None, seed: int = 4):
     """Streaming service: seeded frames (ROI count log-uniform 50-5000, box
@@ -432,6 +467,8 @@ def main(argv=None) -> int:
                     help="frames of the streaming-service run reported beside the headline line (0: skip)")
     ap.add_argument("--boxer-frames", type=int, default=32,
                     help="2048x2048 frames of the ROI-extraction run reported beside the headline line (0: skip)")
+    ap.add_argument("--sim-rois", type=int, default=1_048_576,
+                    help="ROIs of the simulator run reported beside the headline line (0: skip)")
     ap.add_argument("--rois", type=int, default=0, help="ROIs per step (default: workload's)")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "mixed"])
     ap.add_argument("--lanes", type=int, default=0)
@@ -559,6 +596,9 @@ def main(argv=None) -> int:
     roi_extraction = None
     if args.workload == "xynb11" and args.boxer_frames > 0:
         roi_extraction = run_boxer(args.boxer_frames, cpu=not args.no_cpu_baseline)
+    simulator = None
+    if args.workload == "xynb11" and args.sim_rois > 0:
+        simulator = run_simulator(args.sim_rois, cpu=not args.no_cpu_baseline)
 
     rec = {
         "metric": METRIC,
@@ -583,6 +623,7 @@ def main(argv=None) -> int:
         "batched": batched,
         "serving": serving,
         "roi_extraction": roi_extraction,
+        "simulator": simulator,
     }
     print(json.dumps(rec), flush=True)
     return 0
