@@ -1,12 +1,13 @@
 """Generate the polynomial coefficients of the kernel's erf (gmle_erf.hpp).
 
-erf(u), u >= 0, is evaluated in two pieces whose coefficients this script fits:
-  * u <  T : erf(u) = c0 u + u s Q(s), s = u^2   Q of degree DEG_SMALL - 1, c0 = 2/sqrt(pi)
-                                                 (the leading term is applied last in one FMA)
-  * u >= T : erf(u) = 1 - exp(-u^2) * R(u - C)   R of degree DEG_LARGE, R ~ erfcx
-erfcx(u) = exp(u^2) erfc(u) is smooth on [T, U_MAX]; the kernel already has
+erf(u), u >= 0, is evaluated as one piece over [0, U_MAX]:
+    erf(u) = 1 - exp(-u^2) * R(min(u, U_MAX) - C),   R of degree DEG ~ erfcx
+erfcx(u) = exp(u^2) erfc(u) is smooth on [0, U_MAX]; the kernel already has
 exp(-u^2) at every pixel edge (it is the Gaussian of the derivative tables), so
-the large piece costs no extra transcendental.
+erf costs one polynomial and no transcendental of its own.  Beyond U_MAX,
+exp(-u^2) R < 3e-8 and 1 - exp(-u^2) R rounds to 1.0f by itself.  Near u = 0
+the form is accurate in absolute terms (~1e-7) rather than relative ones,
+which is what the pixel integrals Phi = (erf(b) - erf(a)) / 2 consume.
 
 Fitting: least squares on Chebyshev nodes with Lawson's iteratively reweighted
 refinement toward the minimax (equioscillating) relative error, in float64;
@@ -18,11 +19,9 @@ measured on a dense grid.
 import numpy as np
 from scipy.special import erf, erfcx
 
-T = 0.875
 U_MAX = 3.92        # erf(u) rounds to 1.0f beyond this
-DEG_SMALL = 6
-DEG_LARGE = 12
-C_LARGE = (T + U_MAX) / 2
+DEG = 15
+C = U_MAX / 2
 
 
 def lawson_fit(f, x, deg, center, iters=200):
@@ -51,34 +50,18 @@ def horner32(coef32, t32):
 def main():
     m = 20000
     nodes = (np.cos(np.pi * (np.arange(m) + 0.5) / m) + 1) / 2
-    # small piece: erf(u)/u as a polynomial in s = u^2
-    s = nodes * T * T
-    f_small = lambda s: np.where(s > 0, erf(np.sqrt(s)) / np.sqrt(np.maximum(s, 1e-300)), 2 / np.sqrt(np.pi))
-    c0 = 2 / np.sqrt(np.pi)
-    f_q = lambda s: np.where(s > 1e-12, (f_small(s) - c0) / np.maximum(s, 1e-300), -c0 / 3)
-    cq = lawson_fit(lambda s: f_q(s), s, DEG_SMALL - 1, 0.0).astype(np.float32)
-    # large piece: erfcx(u) in (u - C)
-    u = T + nodes * (U_MAX - T)
-    cl = lawson_fit(erfcx, u, DEG_LARGE, C_LARGE).astype(np.float32)
-
-    grid_s = np.linspace(1e-7, T, 400001).astype(np.float32)
-    s32 = (grid_s * grid_s).astype(np.float32)
-    q = horner32(cq, s32)
-    us = (grid_s.astype(np.float64) * s32).astype(np.float32)                    # u * s
-    tail = (us.astype(np.float64) * q.astype(np.float64)).astype(np.float32)     # (u s) Q(s)
-    vs = (grid_s.astype(np.float64) * np.float64(np.float32(c0)) + tail).astype(np.float32)
-    e_small = np.abs(vs.astype(np.float64) / erf(grid_s.astype(np.float64)) - 1).max()
-    grid_l = np.linspace(T, U_MAX, 400001).astype(np.float32)
-    vl = horner32(cl, (grid_l - np.float32(C_LARGE)).astype(np.float32))
-    e_large = np.abs(vl.astype(np.float64) / erfcx(grid_l.astype(np.float64)) - 1).max()
-    print(f"// small piece: max relative error of erf {e_small:.3e}")
-    print(f"// large piece: max relative error of erfcx {e_large:.3e}")
-    print(f"constexpr float kErfT = {T}f;")
+    u = nodes * U_MAX
+    c = lawson_fit(erfcx, u, DEG, C).astype(np.float32)
+    grid = np.linspace(0, U_MAX, 400001).astype(np.float32)
+    r = horner32(c, (grid - np.float32(C)).astype(np.float32))
+    e_rel = np.abs(r.astype(np.float64) / erfcx(grid.astype(np.float64)) - 1).max()
+    g32 = np.exp(-(grid.astype(np.float64) ** 2)).astype(np.float32)
+    v = (np.float32(1) - (g32.astype(np.float64) * r.astype(np.float64)).astype(np.float32)).astype(np.float32)
+    e_abs = np.abs(v.astype(np.float64) - erf(grid.astype(np.float64))).max()
+    print(f"// erfcx: max relative error {e_rel:.3e}; erf: max absolute error {e_abs:.3e}")
     print(f"constexpr float kErfUMax = {U_MAX}f;")
-    print(f"constexpr float kErfC = {float(np.float32(C_LARGE))!r}f;")
-    print(f"constexpr float kErfC0 = {float(np.float32(c0))!r}f;")
-    print("constexpr float kErfSmall[%d] = {%s};" % (len(cq), ", ".join(f"{c!r}f" for c in map(float, cq))))
-    print("constexpr float kErfLarge[%d] = {%s};" % (len(cl), ", ".join(f"{c!r}f" for c in map(float, cl))))
+    print(f"constexpr float kErfC = {float(np.float32(C))!r}f;")
+    print("constexpr float kErfR[%d] = {%s};" % (len(c), ", ".join(f"{x!r}f" for x in map(float, c))))
 
 
 if __name__ == "__main__":

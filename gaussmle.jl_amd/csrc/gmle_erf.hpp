@@ -2,15 +2,17 @@
 //
 // The separable model needs erf and the Gaussian exp(-u^2) at every pixel edge
 // (u = t / (sigma sqrt 2)); the Gaussian is already computed for the derivative
-// tables, so erf is evaluated without a transcendental of its own:
-//   |u| <  kErfT : erf(u) = c0 u + u s Q(s),        s = u^2, c0 = 2/sqrt(pi)
-//   |u| >= kErfT : erf(u) = 1 - exp(-u^2) R(|u| - kErfC),   R ~ erfcx(u) = exp(u^2) erfc(u)
-//   |u| >= kErfUMax : erf(u) = 1 (Float32 erf rounds to 1 there)
-// Both pieces are always evaluated and one is selected (no divergent branch).
-// Q and R are near-minimax polynomials fitted by gen_erf_coeffs.py (Lawson-
-// reweighted least squares on Chebyshev nodes).  Measured Float32 FMA-Horner
-// error: 1.5e-7 relative on the small piece, 1.6e-7 relative on erfcx (at most
-// 5e-8 absolute on erf).  Callable from host code too, for testing.
+// tables, so erf is evaluated without a transcendental of its own, as one
+// piece over the whole range:
+//   erf(u) = sign(u) (1 - exp(-u^2) R(min(|u|, kErfUMax) - kErfC)),  R ~ erfcx
+// (erfcx(u) = exp(u^2) erfc(u)).  Beyond kErfUMax the product is below 3e-8
+// and the result rounds to 1 by itself.  R is a near-minimax polynomial of
+// degree 15 fitted by gen_erf_coeffs.py (Lawson-reweighted least squares on
+// Chebyshev nodes).  Measured Float32 FMA-Horner error: 3.0e-7 relative on
+// erfcx, 2.2e-7 absolute on erf; pixel integrals (erf(b) - erf(a)) / 2 of one
+// pixel are within 1.2e-7 absolute, 7.4e-7 relative for the central ones.
+// Near u = 0 the error is absolute, not relative, which is what the pixel
+// integrals consume.  Callable from host code too, for testing.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -18,42 +20,29 @@
 
 namespace gmle {
 
-constexpr float kErfT = 0.875f;
 constexpr float kErfUMax = 3.92f;
-constexpr float kErfC = 2.3975000381469727f;
-constexpr float kErfC0 = 1.128379225730896f;
+constexpr float kErfC = 1.9600000381469727f;
 constexpr float kErfLog2e = 1.4426950408889634f;
 
 __host__ __device__ __forceinline__ float erf_with_gauss(float u, float g) {
-  const float a = fabsf(u);
-  const float s = a * a;
-  // small piece
-  float q = 0.00016622350085526705f;
-  q = fmaf(q, s, -0.0009764697169885039f);
-  q = fmaf(q, s, 0.005313991568982601f);
-  q = fmaf(q, s, -0.02689233049750328f);
-  q = fmaf(q, s, 0.11284056305885315f);
-  q = fmaf(q, s, -0.3761264383792877f);
-  const float small = fmaf(a, kErfC0, (a * s) * q);
-  // large piece
-  const float x = fminf(a, kErfUMax) - kErfC;
-  float r = 1.405045821911699e-07f;
-  r = fmaf(r, x, -5.948114676357363e-07f);
-  r = fmaf(r, x, 1.3180316500438494e-06f);
-  r = fmaf(r, x, -4.802132934855763e-06f);
-  r = fmaf(r, x, 2.0551316993078217e-05f);
-  r = fmaf(r, x, -7.470209675375372e-05f);
-  r = fmaf(r, x, 0.0002583200694061816f);
-  r = fmaf(r, x, -0.0008767007384449244f);
-  r = fmaf(r, x, 0.002878588391467929f);
-  r = fmaf(r, x, -0.009095106273889542f);
-  r = fmaf(r, x, 0.02756226807832718f);
-  r = fmaf(r, x, -0.07972286641597748f);
-  r = fmaf(r, x, 0.21869787573814392f);
-  float large = fmaf(-g, r, 1.0f);
-  large = a >= kErfUMax ? 1.0f : large;
-  const float v = a < kErfT ? small : large;
-  return copysignf(v, u);
+  const float x = fminf(fabsf(u), kErfUMax) - kErfC;
+  float r = -1.6134004354739773e-08f;
+  r = fmaf(r, x, 7.332921825309313e-08f);
+  r = fmaf(r, x, -5.116217138834145e-08f);
+  r = fmaf(r, x, 6.4967665025506e-08f);
+  r = fmaf(r, x, -1.5845183725105016e-06f);
+  r = fmaf(r, x, 6.3585248426534235e-06f);
+  r = fmaf(r, x, -1.8177152014686726e-05f);
+  r = fmaf(r, x, 5.988409975543618e-05f);
+  r = fmaf(r, x, -0.0002034423960139975f);
+  r = fmaf(r, x, 0.0006487831706181169f);
+  r = fmaf(r, x, -0.0019805324263870716f);
+  r = fmaf(r, x, 0.005820787977427244f);
+  r = fmaf(r, x, -0.016361111775040627f);
+  r = fmaf(r, x, 0.04371161758899689f);
+  r = fmaf(r, x, -0.11021632701158524f);
+  r = fmaf(r, x, 0.2597354054450989f);
+  return copysignf(fmaf(-g, r, 1.0f), u);
 }
 
 }  // namespace gmle
