@@ -275,8 +275,10 @@ void gmle_philox4x32_10(const uint32_t* counter, const uint32_t* key, uint32_t* 
  *     slot's arena (admission by the HBM budget fixed at creation) and
  *     launches the slot's HIP graph: ingest (reads the requests' pinned
  *     buffers directly, converts ADU to electrons with the resident maps),
- *     grouping by shape, one fit kernel per served shape, result scatter
- *     (writes straight into each request's pinned output buffers);
+ *     grouping by shape, the fit (Float32: one multi-shape kernel for every
+ *     served shape; mixed precision: one kernel per shape and its Float64
+ *     final pass), result scatter (writes straight into each request's
+ *     pinned output buffers);
  *   - up to `slots` batches are in flight at once, each on its own stream, so
  *     requests arriving while others run are admitted as slots free up and
  *     their work overlaps on the GPU;
