@@ -59,9 +59,11 @@ def _errors(model_id: int, res: dict, ref32: dict, ref64: dict, precision: str) 
         floor = 1.0 if (k == ibg or (model_id == 3 and k == 2)) else 0.0
         with np.errstate(divide="ignore", invalid="ignore"):
             e[f"p{k}"] = np.abs(gp[:, k] - ref64["params"][:, k]) / np.maximum(floor, np.abs(ref64["params"][:, k]))
-    ce = _rel(gc, ref64["crlb"]).max(axis=1)
-    ce[np.all(np.isinf(gc) & np.isinf(ref64["crlb"]), axis=1)] = 0.0
-    e["crlb"] = ce
+    # CRLB per parameter, relative to the Float64 oracle (both +Inf: agreed)
+    ce = _rel(gc, ref64["crlb"])
+    ce[np.isinf(gc) & np.isinf(ref64["crlb"])] = 0.0
+    for k in range(P):
+        e[f"crlb{k}"] = ce[:, k]
     if precision == "mixed":
         e["llr"] = np.abs(gl - ref64["llr"]) / np.maximum(np.abs(ref64["llr"]), 10.0)
     else:
@@ -146,6 +148,33 @@ def summary(rep: dict) -> str:
                 parts.append(f"{k}:max={fin.max():.2e},p99.9={np.quantile(fin, 0.999):.2e}")
     parts.append(f"ill_pos_max={rep['ill_pos_max']:.2e}")
     return " ".join(parts)
+
+
+PARAM_NAMES = {0: ["x", "y", "N", "bg"], 1: ["x", "y", "N", "bg", "sigma"],
+               2: ["x", "y", "N", "bg", "sigma_x", "sigma_y"], 3: ["x", "y", "z", "N", "bg"]}
+
+
+def distributions(model_id: int, rep: dict) -> dict:
+    """Per output (position, each parameter, each CRLB, LLR): max, p99.9 and
+    the count over tolerance, on the well-posed ROIs and on all ROIs."""
+    names = PARAM_NAMES[model_id]
+    out = {}
+    for k, v in rep.items():
+        if not (isinstance(v, np.ndarray) and v.dtype != bool):
+            continue
+        label = k
+        if k.startswith("crlb"):
+            label = "CRLB " + names[int(k[4:])]
+        elif k.startswith("p") and k[1:].isdigit():
+            label = names[int(k[1:])]
+        elif k == "pos":
+            label = "x, y (px, vs Float32)"
+        tol = POS_TOL if k == "pos" else (LLR_TOL if k == "llr" else REL_TOL)
+        fin = v[np.isfinite(v)]
+        out[label] = {"max": float(fin.max()) if fin.size else float("nan"),
+                      "p99.9": float(np.quantile(fin, 0.999)) if fin.size else float("nan"),
+                      "over_tol": int((~(v <= tol)).sum()), "tol": tol}
+    return out
 
 
 def assert_parity(rep: dict, label: str = ""):

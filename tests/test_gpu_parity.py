@@ -95,8 +95,18 @@ def gpu_fit_device(w, precision, lanes):
     return res
 
 
-def _report(label, rep):
+def _report(label, rep, model_id=None):
+    """Prints the summary; with GMLE_PARITY_REPORT set, also appends the
+    per-output error distributions to that JSON-lines file."""
+    import json
+    import os
     print(f"[parity] {label} {summary(rep)}", flush=True)
+    path = os.environ.get("GMLE_PARITY_REPORT")
+    if path and model_id is not None:
+        from parity import distributions
+        with open(path, "a") as f:
+            f.write(json.dumps({"case": label, "n": rep["n"], "ill_posed": rep["n_ill"], "outliers": rep["n_bad"],
+                                "errors": distributions(model_id, rep)}) + "\n")
 
 
 @pytest.mark.parametrize("name", list(_cases()))
@@ -109,7 +119,7 @@ def test_parity_host_path(name, precision):
         print(f"[case] {name} {precision} lanes={lanes} n={w.n}", flush=True)
         rmix = reference(name, refcpu.MIXED) if precision == "mixed" else None
         rep = compare(w.psf_pixels.model_id, gpu_fit_host(w, precision, lanes), r32, r64, precision, r64p, rmix)
-        _report(f"{name} {precision} L={lanes}", rep)
+        _report(f"{name} {precision} L={lanes}", rep, w.psf_pixels.model_id)
         assert_parity(rep, f"{name} {precision} L={lanes}")
 
 
