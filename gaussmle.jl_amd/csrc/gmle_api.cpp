@@ -96,7 +96,8 @@ int hip_fail(hipError_t e, const char* what) {
 }
 
 bool lanes_ok(int box, int lanes) {
-  return box >= kMinBox && box <= kMaxBox && ((lanes == 1 && box <= 12) || lanes == 2 || lanes == 4);
+  return box >= kMinBox && box <= kMaxBox &&
+         ((lanes == 1 && box <= 12) || lanes == 2 || lanes == 4 || lanes == 8 || lanes == 16);
 }
 
 // Lanes per ROI for a large batch: the fewest lanes whose kernel still runs
@@ -105,16 +106,19 @@ int large_batch_lanes(int model, int camera, int box) {
   return box <= max_box_within_256(model, camera, 1) ? 1 : box <= max_box_within_256(model, camera, 2) ? 2 : 4;
 }
 
-// Lanes per ROI when the caller passes 0: the large-batch choice, raised while
-// the launch has too few waves to fill the chip (more lanes per ROI give more,
-// shorter waves).
+// Lanes per ROI when the caller passes 0: the large-batch choice, raised to
+// the most lanes that keep the launch within one wave per SIMD.  Below that
+// the chip is latency-bound and more, shorter waves finish sooner; above it
+// the extra waves share SIMDs and the replicated per-ROI work costs more than
+// it saves.  Measured on XYNB 11 px and sCMOS XYNBS 13 px: 1K and 4K ROIs run
+// fastest at 16 lanes, 10K at 4, 30K at 2.
 int default_lanes(int model, int camera, int box, int precision, int64_t n_rois) {
   (void)precision;
-  const int64_t fill = 256 * 4 * 2 * 64;   // CUs x SIMDs x waves x lanes
+  const int64_t fill = 256 * 4 * 64;   // CUs x SIMDs x lanes: one wave per SIMD
   int lanes = large_batch_lanes(model, camera, box);
   if (!lanes_ok(box, lanes)) lanes = 2;
-  if (lanes < 2 && n_rois * lanes < fill && lanes_ok(box, 2)) lanes = 2;
-  if (lanes < 4 && n_rois * lanes < fill && lanes_ok(box, 4)) lanes = 4;
+  for (int more = 2; more <= 16; more *= 2)
+    if (lanes < more && n_rois * more <= fill && lanes_ok(box, more)) lanes = more;
   return lanes;
 }
 

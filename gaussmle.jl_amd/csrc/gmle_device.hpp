@@ -1,6 +1,6 @@
 // gmle_device.hpp -- device-side building blocks of the gfx950 fitting kernels.
 //
-// Small helpers shared by every kernel instantiation: wave64 quad reductions
+// Small helpers shared by every kernel instantiation: wave64 group reductions
 // through DPP, fast transcendental wrappers, Julia-compatible clamp/min/max, and
 // the Fisher-matrix inversion (regularised Cholesky, no-pivot LU fallback, Inf)
 // that mirrors src/unified_kernel.jl:7-137,399-431 of GaussMLE.jl.
@@ -14,40 +14,47 @@
 namespace gmle {
 
 // ---------------------------------------------------------------------------
-// Cross-lane sums inside a group of L (1, 2 or 4) lanes of one quad.
-// quad_perm [1,0,3,2] swaps neighbours, [2,3,0,1] swaps pairs.  Every lane of
-// the group ends with the bit-identical sum (a0+a1)+(a2+a3), so replicated
-// per-ROI arithmetic stays identical across the group.
+// Cross-lane sums inside a group of L (1, 2, 4, 8 or 16) lanes of one DPP row.
+// quad_perm [1,0,3,2] swaps neighbours and [2,3,0,1] swaps pairs; after the
+// quad sums, row_half_mirror (lane i <-> 7 - i within 8 lanes) meets the other
+// quad of an 8-lane group and row_mirror (i <-> 15 - i within 16 lanes) the
+// other half of a 16-lane group.  Every step adds two partial sums in either
+// order, so every lane of the group ends with the bit-identical sum and
+// replicated per-ROI arithmetic stays identical across the group.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float dpp_swap1(float v) {
-  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
+constexpr int kDppSwap1 = 0xB1;        // quad_perm [1,0,3,2]
+constexpr int kDppSwap2 = 0x4E;        // quad_perm [2,3,0,1]
+constexpr int kDppHalfMirror = 0x141;  // row_half_mirror
+constexpr int kDppMirror = 0x140;      // row_mirror
+
+template <int CTRL> __device__ __forceinline__ float dpp_move(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
 }
-__device__ __forceinline__ float dpp_swap2(float v) {
-  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, false));
-}
-__device__ __forceinline__ double dpp_swap1(double v) {
+template <int CTRL> __device__ __forceinline__ double dpp_move(double v) {
   const int64_t b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffff), 0xB1, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0xB1, 0xF, 0xF, false);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffff), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, false);
   return __longlong_as_double(((int64_t)hi << 32) | (uint32_t)lo);
 }
-__device__ __forceinline__ double dpp_swap2(double v) {
-  const int64_t b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffff), 0x4E, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0x4E, 0xF, 0xF, false);
-  return __longlong_as_double(((int64_t)hi << 32) | (uint32_t)lo);
-}
+__device__ __forceinline__ float dpp_swap1(float v) { return dpp_move<kDppSwap1>(v); }
+__device__ __forceinline__ float dpp_swap2(float v) { return dpp_move<kDppSwap2>(v); }
+__device__ __forceinline__ double dpp_swap1(double v) { return dpp_move<kDppSwap1>(v); }
+__device__ __forceinline__ double dpp_swap2(double v) { return dpp_move<kDppSwap2>(v); }
 
 template <int L, class T> __device__ __forceinline__ T group_sum(T v) {
-  static_assert(L == 1 || L == 2 || L == 4, "lanes per ROI must be 1, 2 or 4");
-  if constexpr (L >= 2) v = v + dpp_swap1(v);
-  if constexpr (L >= 4) v = v + dpp_swap2(v);
+  static_assert(L == 1 || L == 2 || L == 4 || L == 8 || L == 16, "lanes per ROI must be 1, 2, 4, 8 or 16");
+  if constexpr (L >= 2) v = v + dpp_move<kDppSwap1>(v);
+  if constexpr (L >= 4) v = v + dpp_move<kDppSwap2>(v);
+  if constexpr (L >= 8) v = v + dpp_move<kDppHalfMirror>(v);
+  if constexpr (L >= 16) v = v + dpp_move<kDppMirror>(v);
   return v;
 }
 
 template <int L> __device__ __forceinline__ float group_max(float v) {
-  if constexpr (L >= 2) v = fmaxf(v, dpp_swap1(v));
-  if constexpr (L >= 4) v = fmaxf(v, dpp_swap2(v));
+  if constexpr (L >= 2) v = fmaxf(v, dpp_move<kDppSwap1>(v));
+  if constexpr (L >= 4) v = fmaxf(v, dpp_move<kDppSwap2>(v));
+  if constexpr (L >= 8) v = fmaxf(v, dpp_move<kDppHalfMirror>(v));
+  if constexpr (L >= 16) v = fmaxf(v, dpp_move<kDppMirror>(v));
   return v;
 }
 

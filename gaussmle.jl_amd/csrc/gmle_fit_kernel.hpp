@@ -9,7 +9,8 @@
 // (src/constraints.jl:172-177).
 //
 // Work mapping (wave64, one wave per workgroup):
-//   * L lanes (1, 2 or 4, inside one quad) share one ROI; 64/L ROIs per wave.
+//   * L lanes (1, 2, 4, 8 or 16, inside one DPP row) share one ROI; 64/L ROIs
+//     per wave.
 //   * Lane r of a group owns the contiguous column block [r*CB, r*CB + CB) with
 //     CB = ceil(N/L); its CB*N pixels (and sCMOS variances) are loaded once and
 //     stay in VGPRs for every Newton iteration and the final pass.
@@ -20,7 +21,7 @@
 //     table, so the per-pixel work is one reciprocal plus about ten FMAs that
 //     feed per-column accumulators.  Column totals are folded into the P
 //     gradient and P diagonal-Hessian sums, which are summed across the L lanes
-//     with DPP quad permutes.
+//     with DPP permutes.
 //   * The Newton update (exact division, step clamp, box clamp) is replicated
 //     on the L lanes so they stay bit-identical.
 //   * PREC_FP32 computes the final pass (LLR, full Fisher matrix, inversion,

@@ -2,8 +2,9 @@
 // specialisations built for one (model, camera, precision) triple, and the
 // lookup that maps run-time arguments onto them.
 //
-// Every box size 3..16 is built.  Lanes per ROI: 1, 2 and 4 up to 12 px;
-// 2 and 4 above, where one lane cannot keep a whole ROI resident in VGPRs.
+// Every box size 3..16 is built.  Lanes per ROI: 1, 2, 4, 8 and 16 up to
+// 12 px; 2 to 16 above, where one lane cannot keep a whole ROI resident in
+// VGPRs.  8 and 16 lanes serve small batches: shorter waves, more of them.
 #pragma once
 
 #include "gmle_fit_kernel.hpp"
@@ -12,7 +13,7 @@ namespace gmle {
 
 constexpr bool lanes_built(int box, int lanes) {
   return (box >= kMinBox && box <= kMaxBox) &&
-         ((lanes == 1 && box <= 12) || lanes == 2 || lanes == 4);
+         ((lanes == 1 && box <= 12) || lanes == 2 || lanes == 4 || lanes == 8 || lanes == 16);
 }
 
 // The launcher (Describe = false) or the kernel description (true) of one
@@ -33,6 +34,8 @@ Fn pick_lanes(int lanes) {
   }
   if (lanes == 2) return entry<MODEL, CAM, PREC, N, 2, Describe>();
   if (lanes == 4) return entry<MODEL, CAM, PREC, N, 4, Describe>();
+  if (lanes == 8) return entry<MODEL, CAM, PREC, N, 8, Describe>();
+  if (lanes == 16) return entry<MODEL, CAM, PREC, N, 16, Describe>();
   return nullptr;
 }
 

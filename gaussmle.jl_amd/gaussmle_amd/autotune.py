@@ -1,7 +1,7 @@
 """Autotuner for mixed-shape batches.
 
-Each (model, box) shape has kernels for 1, 2 or 4 lanes per ROI (64, 32 or 16
-ROIs per wavefront; a lane holds a block of ceil(box / L) pixel columns of its
+Each (model, box) shape has kernels for 1, 2, 4, 8 or 16 lanes per ROI (64 down
+to 4 ROIs per wavefront; a lane holds a block of ceil(box / L) pixel columns of its
 ROI in VGPRs, which is the kernel's tile).  The best choice depends on the
 shape's register footprint and on how many ROIs of it a launch has, and in a
 batch also on what runs beside it.  The tuner:
@@ -104,7 +104,7 @@ def tune(mb: gb.MixedBatch, d: dict, precision: str = "fp32", reps: int = 3, sta
     best = {}
     for key in stacks:
         m, b = key
-        ok = [L for L in (1, 2, 4) if lib.gmle_lanes_supported(b, L)]
+        ok = [L for L in (1, 2, 4, 8, 16) if lib.gmle_lanes_supported(b, L)]
         alone[key] = {L: _median_ms(lambda: plain_launch(mb, d, stacks, key, L, precision, outs), reps)
                       for L in ok}
         best[key] = min(alone[key], key=alone[key].get)

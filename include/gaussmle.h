@@ -83,11 +83,13 @@ int gmle_default_constraints(int model, int box, float* lower, float* upper, flo
 /* Largest box size supported by the kernels (smallest is 3). */
 int gmle_max_box(void);
 
-/* 1 when a kernel with this many lanes per ROI exists for the box size. */
+/* 1 when a kernel with this many lanes per ROI exists for the box size:
+ * 2, 4, 8 and 16 for every box, 1 up to 12 px. */
 int gmle_lanes_supported(int box, int lanes);
 
 /* Lanes per ROI chosen when a fit call passes lanes = 0: the library's tuned
- * choice for this shape and batch size. */
+ * choice for this shape and batch size (more lanes for smaller batches, so
+ * that the launch still fills the chip). */
 int gmle_default_lanes(int model, int camera, int box, int precision, int64_t n_rois);
 
 /* Replaces one reference kernel launch plus the GPU batch loop

@@ -44,8 +44,12 @@ def test_scalar_queries(hiplib):
     assert hiplib.gmle_lanes_supported(15, 1) == 0
     assert hiplib.gmle_lanes_supported(15, 4) == 1
     assert hiplib.gmle_lanes_supported(2, 1) == 0
+    assert hiplib.gmle_lanes_supported(16, 8) == 1 and hiplib.gmle_lanes_supported(16, 16) == 1
+    assert hiplib.gmle_lanes_supported(11, 32) == 0
     assert hiplib.gmle_default_lanes(0, 0, 11, 0, 10_000_000) == 1
-    assert hiplib.gmle_default_lanes(0, 0, 11, 0, 100) in (2, 4)
+    assert hiplib.gmle_default_lanes(0, 0, 11, 0, 100) == 16
+    assert hiplib.gmle_default_lanes(0, 0, 11, 0, 10_000) == 4
+    assert hiplib.gmle_default_lanes(0, 0, 11, 0, 30_000) == 2
     assert hiplib.gmle_device_count() >= 0
 
 

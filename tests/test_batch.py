@@ -39,7 +39,7 @@ def lanes_table(mb, choice):
         if c == 0:
             continue
         m, b = gb.shape_of(s)
-        ok = [L for L in (1, 2, 4) if lib.gmle_lanes_supported(b, L)]
+        ok = [L for L in (1, 2, 4, 8, 16) if lib.gmle_lanes_supported(b, L)]
         t[s] = 0 if choice == "default" else (max(ok) if choice == "max" else min(ok))
     return t
 

@@ -63,7 +63,7 @@ def reference(name, prec):
 
 
 def lanes_for(box):
-    return [l for l in (1, 2, 4) if _lib.load().gmle_lanes_supported(box, l)]
+    return [l for l in (1, 2, 4, 8, 16) if _lib.load().gmle_lanes_supported(box, l)]
 
 
 def gpu_fit_host(w, precision, lanes):
