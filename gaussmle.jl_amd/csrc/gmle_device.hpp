@@ -58,6 +58,46 @@ template <int L> __device__ __forceinline__ float group_max(float v) {
   return v;
 }
 
+// Moves inside a group of L lanes (lane r of a group is lane r of its ROI):
+//   group_next<L>(v):     lane r receives lane (r + 1) mod L's v
+//   group_bcast<L, O>(v): every lane receives lane O's v
+// quad_perm patterns for 2 and 4 lanes, row_ror / row_newbcast for 16 (one
+// DPP row), and two row moves plus a select for the half rows of 8.
+constexpr int kDppRowShr = 0x110;       // row_shr:n = 0x110 + n, lane i <- i - n
+constexpr int kDppRowRor = 0x120;       // row_ror:n = 0x120 + n, lane i <- (i - n) mod 16
+constexpr int kDppRowNewBcast = 0x150;  // row_newbcast:n = 0x150 + n, lane i <- lane n of its row
+
+template <int L> __device__ __forceinline__ float group_next(float v) {
+  static_assert(L == 2 || L == 4 || L == 8 || L == 16, "groups of 2, 4, 8 or 16 lanes");
+  if constexpr (L == 2) {
+    return dpp_move<kDppSwap1>(v);
+  } else if constexpr (L == 4) {
+    return dpp_move<0x39>(v);   // quad_perm [1,2,3,0]
+  } else if constexpr (L == 16) {
+    return dpp_move<kDppRowRor + 15>(v);
+  } else {
+    const float a = dpp_move<kDppRowRor + 15>(v);   // lane i <- (i + 1) mod 16
+    const float b = dpp_move<kDppRowShr + 7>(v);    // lanes 7 and 15 <- lanes 0 and 8
+    return (threadIdx.x & 7) == 7 ? b : a;
+  }
+}
+
+template <int L, int O> __device__ __forceinline__ float group_bcast(float v) {
+  static_assert(L == 2 || L == 4 || L == 8 || L == 16, "groups of 2, 4, 8 or 16 lanes");
+  static_assert(O >= 0 && O < L, "source lane inside the group");
+  if constexpr (L == 2) {
+    return dpp_move<O | (O << 2) | ((2 + O) << 4) | ((2 + O) << 6)>(v);
+  } else if constexpr (L == 4) {
+    return dpp_move<O * 0x55>(v);   // quad_perm [O,O,O,O]
+  } else if constexpr (L == 16) {
+    return dpp_move<kDppRowNewBcast + O>(v);
+  } else {
+    const float a = dpp_move<kDppRowNewBcast + O>(v);
+    const float b = dpp_move<kDppRowNewBcast + 8 + O>(v);
+    return (threadIdx.x & 8) ? b : a;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Transcendentals: exp2 is a single v_exp_f32, 1/x a single v_rcp_f32 (1 ulp).
 // ---------------------------------------------------------------------------

@@ -181,6 +181,94 @@ __device__ __forceinline__ Pix<T> pix_from_edges(const AxisK<T>& k, const Edge<T
   return p;
 }
 
+// ---------------------------------------------------------------------------
+// Row tables shared by the L lanes of an ROI.  Every lane needs every row's
+// tables, but evaluating the N + 1 row edges (one erf and one exp each) on
+// all L lanes repeats the same work L times.  Instead lane r evaluates edges
+// r, r + L, r + 2L, ... (slot s holds edge r + L s) and builds the tables of
+// rows r, r + L, ...; a row's tables reach the other lanes by a DPP
+// broadcast from its owner where they are used.  Row r + L s's upper edge is
+// lane r + 1's slot-s edge, or lane 0's slot-(s + 1) edge for the group's
+// last lane: lane 0 offers its next edge, the others their current one, and
+// every lane reads the next lane's offer.  Each value comes from the same
+// expression as in the replicated evaluation, so the tables are bit-identical.
+// ---------------------------------------------------------------------------
+// Where sharing is used: 4 or more lanes per ROI, except the largest sCMOS
+// and astigmatic tiles, where its few extra live values would push the kernel
+// past 256 VGPRs (one wave per SIMD instead of two, gfx950 register counts).
+// With 2 lanes it halves the edge work but costs a box size of the 2-wave
+// range, so 2-lane kernels keep the replicated evaluation.
+template <int MODEL, int CAM, int N, int L> constexpr bool share_rows() {
+  if (L < 4) return false;
+  if (L > 4) return true;
+  if (N == 16 && (MODEL == ASTIG || (CAM == SCMOS && MODEL != XYNB))) return false;
+  if (MODEL == ASTIG && CAM == SCMOS && N >= 13) return false;
+  return true;
+}
+
+template <int N, int L> constexpr int row_slots() { return (N + L - 1) / L; }   // rows per lane
+template <int N, int L> constexpr int edge_slots() { return (N + L) / L; }     // edges 0..N per lane
+
+template <int L, bool SIG>
+__device__ __forceinline__ Pix<float> slot_row(const AxisK<float>& kr, int r, const Edge<float>& e_s,
+                                               const Edge<float>& e_next) {
+  const bool first = r == 0;
+  Edge<float> hi;
+  hi.ev = group_next<L>(first ? e_next.ev : e_s.ev);
+  hi.g = group_next<L>(first ? e_next.g : e_s.g);
+  hi.tg = group_next<L>(first ? e_next.tg : e_s.tg);
+  hi.t3g = SIG ? group_next<L>(first ? e_next.t3g : e_s.t3g) : 0.f;
+  return pix_from_edges<float, SIG>(kr, e_s, hi);
+}
+
+// The edge of slot s + 1 when some lane has one (else unused).
+template <int N, int L, bool SIG>
+__device__ __forceinline__ Edge<float> next_slot_edge(const AxisK<float>& kr, int r, int s, const Edge<float>& e_s) {
+  return (s + 1 < edge_slots<N, L>()) ? edge_eval<float, SIG>(kr, r + L * (s + 1)) : e_s;
+}
+
+// Row i's tables on every lane of the group, from lane o = i mod L.  o is a
+// constant once the caller's row loop is unrolled, so the chain of tests
+// folds to a single broadcast.
+template <int L, bool SIG, int O = 0>
+__device__ __forceinline__ Pix<float> bcast_pix(const Pix<float>& p, int o) {
+  if constexpr (O + 1 < L) {
+    if (o != O) return bcast_pix<L, SIG, O + 1>(p, o);
+  }
+  Pix<float> q;
+  q.phi = group_bcast<L, O>(p.phi);
+  q.D = group_bcast<L, O>(p.D);
+  q.D2 = group_bcast<L, O>(p.D2);
+  q.S = SIG ? group_bcast<L, O>(p.S) : 0.f;
+  q.SS = SIG ? group_bcast<L, O>(p.SS) : 0.f;
+  return q;
+}
+
+// All N rows' tables on every lane, as axis_tables() builds them.
+template <int N, int L, bool SIG>
+__device__ __forceinline__ void shared_axis_tables(const AxisK<float>& kr, int r, AxisTab<float, N>& tb) {
+  constexpr int RS = row_slots<N, L>();
+  Pix<float> own[RS];
+  Edge<float> e_s = edge_eval<float, SIG>(kr, r);
+#pragma unroll
+  for (int s = 0; s < RS; ++s) {
+    const Edge<float> e_next = next_slot_edge<N, L, SIG>(kr, r, s, e_s);
+    own[s] = slot_row<L, SIG>(kr, r, e_s, e_next);
+    e_s = e_next;
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const Pix<float> q = bcast_pix<L, SIG>(own[i / L], i % L);
+    tb.phi[i] = q.phi;
+    tb.D[i] = q.D;
+    tb.D2[i] = q.D2;
+    if constexpr (SIG) {
+      tb.S[i] = q.S;
+      tb.SS[i] = q.SS;
+    }
+  }
+}
+
 // Per-ROI width parameters for one evaluation of the model.
 template <class T> struct Widths {
   T sx, sy;          // sigma per axis (pixels)
@@ -272,7 +360,7 @@ __device__ __forceinline__ void fold_column(const Pix<float>& cx, const Widths<f
 // when there are many row tables (sigma / z models) and few columns per lane.
 template <int MODEL, int CAM, int N, int L, bool FAST>
 __device__ __forceinline__ void accumulate_rows(const float* dv, const float* vr, int c0, float nph, float bg,
-                                                const AxisK<float>& kr, const AxisK<float>& kc,
+                                                const AxisK<float>& kr, int r, const AxisK<float>& kc,
                                                 const Widths<float>& w, float* G, float* H) {
   using Tr = MT<MODEL>;
   constexpr int P = Tr::P;
@@ -301,15 +389,30 @@ __device__ __forceinline__ void accumulate_rows(const float* dv, const float* vr
 #pragma unroll
     for (int q = 0; q < 11; ++q) S[cc][q] = 0.f;
   float sphi = 0.f, sD = 0.f, sD2 = 0.f, sA = 0.f, sAA = 0.f;
-  Edge<float> rlo = edge_eval<float, SIG>(kr, 0);
+  // Without sharing, rlo is the running lower edge.  With it, rlo is this
+  // lane's edge of the current slot, and `own` the tables of its row in that
+  // slot, built when the group reaches the slot's first row.
+  constexpr bool SHARE = share_rows<MODEL, CAM, N, L>();
+  Edge<float> rlo = edge_eval<float, SIG>(kr, SHARE ? r : 0);
+  Pix<float> own;
 #pragma unroll
   for (int i = 0; i < N; ++i) {
     // Keep each row's work together: without this the scheduler hoists later
     // rows' edge evaluations and every row's tables become live at once.
     __builtin_amdgcn_sched_barrier(0);
-    const Edge<float> rhi = edge_eval<float, SIG>(kr, i + 1);
-    const Pix<float> ry = pix_from_edges<float, SIG>(kr, rlo, rhi);
-    rlo = rhi;
+    Pix<float> ry;
+    if constexpr (!SHARE) {
+      const Edge<float> rhi = edge_eval<float, SIG>(kr, i + 1);
+      ry = pix_from_edges<float, SIG>(kr, rlo, rhi);
+      rlo = rhi;
+    } else {
+      if (i % L == 0) {
+        const Edge<float> e_next = next_slot_edge<N, L, SIG>(kr, r, i / L, rlo);
+        own = slot_row<L, SIG>(kr, r, rlo, e_next);
+        rlo = e_next;
+      }
+      ry = bcast_pix<L, SIG>(own, i % L);
+    }
     float ay = 0.f, aay = 0.f;
     if constexpr (SIG) {
       if constexpr (MODEL == ASTIG) {
@@ -361,6 +464,17 @@ __device__ __forceinline__ void accumulate_rows(const float* dv, const float* vr
         S[cc][8] = fmaf(aay, cf, S[cc][8]);
         S[cc][10] = fmaf(a2, df, S[cc][10]);
         if constexpr (TWO) S[cc][9] = fmaf(pa, df, S[cc][9]);
+      }
+    }
+    // With shared rows, also tie this row's sums to the end of the row, or
+    // the pixel work of every row sinks below the last row's broadcast.
+    if constexpr (SHARE) {
+#pragma unroll
+      for (int cc = 0; cc < CB; ++cc) {
+        asm volatile("" : "+v"(S[cc][0]), "+v"(S[cc][1]), "+v"(S[cc][2]), "+v"(S[cc][3]), "+v"(S[cc][4]),
+                     "+v"(S[cc][5]), "+v"(S[cc][6]));
+        if constexpr (SIG) asm volatile("" : "+v"(S[cc][7]), "+v"(S[cc][8]), "+v"(S[cc][10]));
+        if constexpr (TWO) asm volatile("" : "+v"(S[cc][9]));
       }
     }
   }
@@ -801,17 +915,21 @@ __device__ __forceinline__ void fit_one(const FitArgs& a, const RoiRef& ref, boo
     }
 
     float G[P], H[P];
+    const AxisK<float> kr = axis_consts<float>(th[Tr::IY], w.sy, nph);
     if constexpr (row_outer<MODEL, CAM, N, L>()) {
-      const AxisK<float> kr = axis_consts<float>(th[Tr::IY], w.sy, nph);
       const AxisK<float> kc = axis_consts<float>(th[Tr::IX], w.sx, nph);
       if (fast) {
-        accumulate_rows<MODEL, CAM, N, L, true>(dv, vr, c0, nph, bg, kr, kc, w, G, H);
+        accumulate_rows<MODEL, CAM, N, L, true>(dv, vr, c0, nph, bg, kr, r, kc, w, G, H);
       } else {
-        accumulate_rows<MODEL, CAM, N, L, false>(dv, vr, c0, nph, bg, kr, kc, w, G, H);
+        accumulate_rows<MODEL, CAM, N, L, false>(dv, vr, c0, nph, bg, kr, r, kc, w, G, H);
       }
     } else {
       AxisTab<float, N> ry;
-      axis_tables<float, N, SIG>(th[Tr::IY], w.sy, nph, 0, ry);
+      if constexpr (share_rows<MODEL, CAM, N, L>()) {
+        shared_axis_tables<N, L, SIG>(kr, r, ry);
+      } else {
+        axis_tables<float, N, SIG>(th[Tr::IY], w.sy, nph, 0, ry);
+      }
       const AxisK<float> kc = axis_consts<float>(th[Tr::IX], w.sx, nph);
 
       // Row extras: A = d(model)/d(extra) on the row axis, AA its second derivative.
