@@ -191,19 +191,21 @@ __device__ __forceinline__ Pix<T> pix_from_edges(const AxisK<T>& k, const Edge<T
 // lane r + 1's slot-s edge, or lane 0's slot-(s + 1) edge for the group's
 // last lane: lane 0 offers its next edge, the others their current one, and
 // every lane reads the next lane's offer.  Each value comes from the same
-// expression as in the replicated evaluation, so the tables are bit-identical.
+// expressions as in the replicated evaluation; the compiler's FMA contraction
+// can still round a rare one differently, so a kernel and its replicated twin
+// may differ in the last bit of an ROI's result.
 // ---------------------------------------------------------------------------
-// Where sharing is used: 4 or more lanes per ROI, except the largest sCMOS
-// and astigmatic tiles, where its few extra live values would push the kernel
-// past 256 VGPRs (one wave per SIMD instead of two, gfx950 register counts).
-// With 2 lanes it halves the edge work but costs a box size of the 2-wave
-// range, so 2-lane kernels keep the replicated evaluation.
+// Where sharing is used: from 4 lanes up, except in the largest 4-lane tile
+// of each model and camera that still fits 256 VGPRs (two waves per SIMD,
+// max_box_within_256): sharing keeps a few more values live, which would push
+// exactly those tiles past it (gfx950 register counts).  2-lane kernels keep
+// the replicated evaluation: there sharing would only halve the edge work,
+// and the multi-shape kernel, which holds every 1-, 2- and 4-lane tile in one
+// function, would no longer fit two waves per SIMD.
 template <int MODEL, int CAM, int N, int L> constexpr bool share_rows() {
   if (L < 4) return false;
-  if (L > 4) return true;
-  if (N == 16 && (MODEL == ASTIG || (CAM == SCMOS && MODEL != XYNB))) return false;
-  if (MODEL == ASTIG && CAM == SCMOS && N >= 13) return false;
-  return true;
+  if (L >= 8) return true;
+  return N != max_box_within_256(MODEL, CAM, L);
 }
 
 template <int N, int L> constexpr int row_slots() { return (N + L - 1) / L; }   // rows per lane
@@ -358,7 +360,7 @@ __device__ __forceinline__ void fold_column(const Pix<float>& cx, const Widths<f
 // the fly from its two edges and the CB columns of this lane keep their S
 // vectors live instead.  Cheaper in VGPRs than holding every row's tables
 // when there are many row tables (sigma / z models) and few columns per lane.
-template <int MODEL, int CAM, int N, int L, bool FAST>
+template <int MODEL, int CAM, int N, int L, bool FAST, bool SHARE>
 __device__ __forceinline__ void accumulate_rows(const float* dv, const float* vr, int c0, float nph, float bg,
                                                 const AxisK<float>& kr, int r, const AxisK<float>& kc,
                                                 const Widths<float>& w, float* G, float* H) {
@@ -392,7 +394,6 @@ __device__ __forceinline__ void accumulate_rows(const float* dv, const float* vr
   // Without sharing, rlo is the running lower edge.  With it, rlo is this
   // lane's edge of the current slot, and `own` the tables of its row in that
   // slot, built when the group reaches the slot's first row.
-  constexpr bool SHARE = share_rows<MODEL, CAM, N, L>();
   Edge<float> rlo = edge_eval<float, SIG>(kr, SHARE ? r : 0);
   Pix<float> own;
 #pragma unroll
@@ -916,16 +917,17 @@ __device__ __forceinline__ void fit_one(const FitArgs& a, const RoiRef& ref, boo
 
     float G[P], H[P];
     const AxisK<float> kr = axis_consts<float>(th[Tr::IY], w.sy, nph);
+    constexpr bool SHARE = share_rows<MODEL, CAM, N, L>();
     if constexpr (row_outer<MODEL, CAM, N, L>()) {
       const AxisK<float> kc = axis_consts<float>(th[Tr::IX], w.sx, nph);
       if (fast) {
-        accumulate_rows<MODEL, CAM, N, L, true>(dv, vr, c0, nph, bg, kr, r, kc, w, G, H);
+        accumulate_rows<MODEL, CAM, N, L, true, SHARE>(dv, vr, c0, nph, bg, kr, r, kc, w, G, H);
       } else {
-        accumulate_rows<MODEL, CAM, N, L, false>(dv, vr, c0, nph, bg, kr, r, kc, w, G, H);
+        accumulate_rows<MODEL, CAM, N, L, false, SHARE>(dv, vr, c0, nph, bg, kr, r, kc, w, G, H);
       }
     } else {
       AxisTab<float, N> ry;
-      if constexpr (share_rows<MODEL, CAM, N, L>()) {
+      if constexpr (SHARE) {
         shared_axis_tables<N, L, SIG>(kr, r, ry);
       } else {
         axis_tables<float, N, SIG>(th[Tr::IY], w.sy, nph, 0, ry);
