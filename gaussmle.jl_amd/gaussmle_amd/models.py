@@ -249,3 +249,8 @@ class ROIBatch:
 
     def __len__(self) -> int:
         return self.data.shape[2]
+
+    @property
+    def roi_size(self) -> int:
+        """Side of the square ROIs (SMLMData.ROIBatch.roi_size)."""
+        return self.data.shape[0]

@@ -128,6 +128,9 @@ class BasicSMLD:
     n_datasets: int
     metadata: dict = field(default_factory=dict)
 
+    def __len__(self) -> int:
+        return len(self.emitters)
+
 
 def to_emitter(result: LocalizationResult, idx: int, camera, dataset: int = 1, track_id: int = 0, id: int | None = None):
     """src/roi_batch.jl:121-302 (idx is 0-based here; id defaults to idx + 1)."""
