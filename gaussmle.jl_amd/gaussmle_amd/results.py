@@ -163,9 +163,48 @@ def to_emitter(result: LocalizationResult, idx: int, camera, dataset: int = 1, t
     raise TypeError(f"unknown PSF model {m!r}")
 
 
+def _emitters(result: LocalizationResult, camera, dataset: int = 1, track_id: int = 0) -> list:
+    """Every emitter of `result`, as to_emitter builds them one at a time: the
+    same Float32 roundings, computed on whole columns first."""
+    psx = camera.pixel_edges_x[1] - camera.pixel_edges_x[0]
+    psy = camera.pixel_edges_y[1] - camera.pixel_edges_y[0]
+    n = result.n_fits
+    p, u = result.parameters, result.uncertainties
+
+    def sx(v):
+        return (v * psx).astype(F32).tolist()
+
+    def sy(v):
+        return (v * psy).astype(F32).tolist()
+
+    def raw(v):
+        return v.tolist()
+
+    x_um = ((result.x_camera - F32(1)) * psx).astype(F32).tolist()
+    y_um = ((result.y_camera - F32(1)) * psy).astype(F32).tolist()
+    tail = (np.asarray(result.pvalues).tolist(), np.asarray(result.frame_indices).astype(int).tolist(),
+            [dataset] * n, [track_id] * n, list(range(1, n + 1)))
+    m = result.psf_model
+    if isinstance(m, GaussianXYNB):
+        cls, cols = Emitter2DFitGaussMLE, (x_um, y_um, raw(p[2]), raw(p[3]), sx(u[0]), sy(u[1]), raw(u[2]),
+                                           raw(u[3]))
+    elif isinstance(m, GaussianXYNBS):
+        cls, cols = Emitter2DFitSigma, (x_um, y_um, raw(p[2]), raw(p[3]), sx(p[4]), sx(u[0]), sy(u[1]), raw(u[2]),
+                                        raw(u[3]), sx(u[4]))
+    elif isinstance(m, GaussianXYNBSXSY):
+        cls, cols = Emitter2DFitSigmaXY, (x_um, y_um, raw(p[2]), raw(p[3]), sx(p[4]), sy(p[5]), sx(u[0]), sy(u[1]),
+                                          raw(u[2]), raw(u[3]), sx(u[4]), sy(u[5]))
+    elif isinstance(m, AstigmaticXYZNB):
+        cls, cols = Emitter3DFitGaussMLE, (x_um, y_um, raw(p[2]), raw(p[3]), raw(p[4]), sx(u[0]), sy(u[1]),
+                                           raw(u[2]), raw(u[3]), raw(u[4]))
+    else:
+        raise TypeError(f"unknown PSF model {m!r}")
+    return [cls(*f) for f in zip(*cols, *tail)]
+
+
 def to_smld(result: LocalizationResult, batch: ROIBatch, dataset: int = 1, metadata: dict | None = None) -> BasicSMLD:
     """src/roi_batch.jl:326-353."""
-    emitters = [to_emitter(result, i, batch.camera, dataset=dataset) for i in range(result.n_fits)]
+    emitters = _emitters(result, batch.camera, dataset=dataset)
     md = dict(metadata or {})
     md["psf_model"] = type(result.psf_model).__name__
     md["n_fits"] = result.n_fits

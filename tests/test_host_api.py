@@ -117,3 +117,27 @@ def test_fitter_defaults_mirror_reference():
     np.testing.assert_array_equal(c.lower, np.float32([-2, -2, 1, 0.01]))
     np.testing.assert_array_equal(c.upper, np.float32([13, 13, np.inf, np.inf]))
     np.testing.assert_array_equal(c.max_step, np.float32([1, 1, 100, 2]))
+
+
+@pytest.mark.parametrize("pixel_dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("model", ["xynb", "xynbs", "xynbsxsy", "astig"])
+def test_to_smld_matches_per_emitter_conversion(model, pixel_dtype):
+    """to_smld converts whole columns at once; every emitter must equal the
+    one to_emitter builds alone (src/roi_batch.jl:121-353)."""
+    from types import SimpleNamespace
+    from gaussmle_amd.models import (AstigmaticXYZNB, GaussianXYNB, GaussianXYNBS, GaussianXYNBSXSY, IdealCamera,
+                                     ROIBatch)
+    from gaussmle_amd.results import LocalizationResult, to_emitter, to_smld
+    psf = {"xynb": GaussianXYNB(0.13), "xynbs": GaussianXYNBS(), "xynbsxsy": GaussianXYNBSXSY(),
+           "astig": AstigmaticXYZNB(0.13, 0.13, 0.05, -0.05, 0.01, -0.01, 0.2, 0.5)}[model]
+    n, P = 500, len(psf)
+    rng = np.random.default_rng(3)
+    out = SimpleNamespace(params=(rng.random((P, n)) * 10).astype(np.float32),
+                          crlb=rng.random((P, n)).astype(np.float32), llr=rng.random(n).astype(np.float32))
+    cam = IdealCamera.square(2048, 2048, 0.1, dtype=pixel_dtype)
+    batch = ROIBatch(np.zeros((11, 11, n), np.float32), rng.integers(1, 2000, n), rng.integers(1, 2000, n),
+                     rng.integers(1, 50, n), cam)
+    loc = LocalizationResult.create(out, rng.random(n), batch, psf)
+    smld = to_smld(loc, batch)
+    assert smld.emitters == [to_emitter(loc, i, cam) for i in range(n)]
+    assert len(smld) == n
