@@ -593,6 +593,10 @@ def main(argv=None) -> int:
     serving = None
     if args.workload == "xynb11" and args.serve_frames > 0:
         serving = run_service(args.serve_frames)
+        # the same service with 16 lanes per ROI: shorter fits for small
+        # batches (latency), at the cost of peak throughput
+        low = run_service(args.serve_frames, loads=(0.1,), lanes=16)
+        serving["low_latency"] = {k: low[k] for k in ("lanes", "peak_fits_per_s", "loads")}
     roi_extraction = None
     if args.workload == "xynb11" and args.boxer_frames > 0:
         roi_extraction = run_boxer(args.boxer_frames, cpu=not args.no_cpu_baseline)

@@ -19,13 +19,13 @@ __host__ __device__ constexpr int n_params(int model) {
   return model == XYNB ? 4 : model == XYNBS ? 5 : model == XYNBSXSY ? 6 : 5;
 }
 
-// Largest box size whose kernel with `lanes` lanes per ROI (1 or 2) still
-// fits 256 VGPRs without spilling, i.e. runs two or more waves per SIMD, with
-// a few registers to spare for the multi-shape kernel's call.  A
-// lane keeps its ceil(box / lanes) pixel columns (and, for sCMOS, their
-// variances) in VGPRs for the whole fit.  From the gfx950 build's register
-// counts (kernel-resource-usage); 4 lanes fit every box except astigmatic
-// sCMOS ROIs of 14 px and up.
+// Largest box size whose kernel with `lanes` lanes per ROI still fits 256
+// VGPRs without spilling, i.e. runs two or more waves per SIMD, with a few
+// registers to spare for the multi-shape kernel's call.  A lane keeps its
+// ceil(box / lanes) pixel columns (and, for sCMOS, their variances) in VGPRs
+// for the whole fit.  From the gfx950 build's register counts
+// (kernel-resource-usage): 4 lanes fit every box except astigmatic sCMOS ROIs
+// of 14 px and up, 8 lanes every box but 16 px outside XYNB, 16 lanes all.
 __host__ __device__ constexpr int max_box_within_256(int model, int camera, int lanes) {
   const bool sc = camera == SCMOS;
   if (lanes == 1)
@@ -34,7 +34,9 @@ __host__ __device__ constexpr int max_box_within_256(int model, int camera, int 
   if (lanes == 2)
     return model == XYNB ? (sc ? 11 : 14) : model == XYNBS ? (sc ? 9 : 11) : model == XYNBSXSY ? (sc ? 9 : 11)
                                                                                                : (sc ? 8 : 10);
-  return (model == ASTIG && sc) ? 13 : kMaxBox;
+  if (lanes == 4) return (model == ASTIG && sc) ? 13 : kMaxBox;
+  if (lanes == 8) return model == XYNB ? kMaxBox : 15;
+  return kMaxBox;
 }
 
 // Passed by value to every fitting kernel.  Arrays follow the reference's

@@ -6,7 +6,8 @@
 // are separate, non-inlined functions, so each is register-allocated on its
 // own: the kernel needs the largest of them (at most 256 VGPRs, two waves per
 // SIMD) instead of what one body holding every shape would need.  Each shape
-// offers its 4-lane tile and, where they fit 256 VGPRs, 2- and 1-lane tiles.
+// offers its 4-lane tile and, where they fit 256 VGPRs, 1-, 2-, 8- and
+// 16-lane tiles (8 and 16 lanes give small batches more, shorter waves).
 // The per-ROI code is fit_one, as in the per-shape kernels, so results are
 // bit-identical to a per-shape launch with the same lanes per ROI.
 #pragma once
@@ -20,7 +21,8 @@ namespace gmle {
 // within 256 VGPRs (two waves per SIMD).  Shapes without a 4-lane tile here
 // (astigmatic sCMOS ROIs of 14 px and up) keep their own launch.
 __host__ __device__ constexpr bool multi_has(int model, int camera, int box, int lanes) {
-  return (lanes == 1 || lanes == 2 || lanes == 4) && box >= kMinBox && box <= max_box_within_256(model, camera, lanes);
+  return (lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8 || lanes == 16) && box >= kMinBox &&
+         box <= max_box_within_256(model, camera, lanes);
 }
 __host__ __device__ constexpr bool multi_eligible(int model, int camera, int box) {
   return multi_has(model, camera, box, 4);
@@ -76,6 +78,12 @@ __device__ __forceinline__ void multi_case(const MultiArgs& m, int q, int t) {
   }
   if constexpr (multi_has(MODEL, CAM, N, 2)) {
     if (L == 2) { multi_tile<MODEL, CAM, N, 2>(m.fa[MODEL], m.first[q], m.count[q], t); return; }
+  }
+  if constexpr (multi_has(MODEL, CAM, N, 8)) {
+    if (L == 8) { multi_tile<MODEL, CAM, N, 8>(m.fa[MODEL], m.first[q], m.count[q], t); return; }
+  }
+  if constexpr (multi_has(MODEL, CAM, N, 16)) {
+    if (L == 16) { multi_tile<MODEL, CAM, N, 16>(m.fa[MODEL], m.first[q], m.count[q], t); return; }
   }
   if constexpr (multi_has(MODEL, CAM, N, 4)) {
     if (L == 4) multi_tile<MODEL, CAM, N, 4>(m.fa[MODEL], m.first[q], m.count[q], t);

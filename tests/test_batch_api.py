@@ -176,11 +176,13 @@ def test_one_launch_lanes(hiplib):
                     continue
                 assert auto in (1, 2, 4)
                 assert hiplib.gmle_one_launch_lanes(m, cam, b, 4) == 4
-                for want in (1, 2):
+                for want in (1, 2, 8, 16):
                     got = hiplib.gmle_one_launch_lanes(m, cam, b, want)
                     assert got in (want, 4)
                     if got == want:   # a smaller box never loses a variant a larger one has
                         assert all(hiplib.gmle_one_launch_lanes(m, cam, bb, want) == want for bb in range(3, b))
     assert hiplib.gmle_one_launch_lanes(0, 0, 11, 1) == 1
+    assert hiplib.gmle_one_launch_lanes(1, 1, 13, 16) == 16 and hiplib.gmle_one_launch_lanes(1, 1, 13, 8) == 8
+    assert hiplib.gmle_one_launch_lanes(1, 0, 16, 8) == 4     # 16 px sigma models: no 8-lane tile within 256
     assert hiplib.gmle_one_launch_lanes(9, 0, 11, 1) == 0
 
