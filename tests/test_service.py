@@ -25,11 +25,17 @@ def frames():
     return _STATE["maps"], _STATE["frames"]
 
 
-@pytest.fixture
-def service():
+LANE_TABLES = {
+    "lanes_1_2_4": {(0, 9): 1, (0, 11): 2, (0, 13): 4, (0, 15): 2, (1, 9): 2, (1, 11): 4, (1, 13): 2, (1, 15): 4},
+    "lanes_8_16": {(0, 9): 16, (0, 11): 8, (0, 13): 16, (0, 15): 8, (1, 9): 16, (1, 11): 16, (1, 13): 8,
+                   (1, 15): 16},
+}
+
+
+@pytest.fixture(params=list(LANE_TABLES))
+def service(request):
     maps, _ = frames()
-    psfs, cons, table = sv.default_setup(lanes={(0, 9): 1, (0, 11): 2, (0, 13): 4, (0, 15): 2,
-                                                (1, 9): 2, (1, 11): 4, (1, 13): 2, (1, 15): 4})
+    psfs, cons, table = sv.default_setup(lanes=LANE_TABLES[request.param])
     s = sv.Service(psfs, cons, table, hbm_budget=1 << 30, slots=3)
     s.calibrate(*maps)
     yield s
