@@ -95,10 +95,7 @@ int hip_fail(hipError_t e, const char* what) {
   return fail(e == hipErrorOutOfMemory ? GMLE_ERR_NOMEM : GMLE_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
 }
 
-bool lanes_ok(int box, int lanes) {
-  return box >= kMinBox && box <= kMaxBox &&
-         ((lanes == 1 && box <= 12) || lanes == 2 || lanes == 4 || lanes == 8 || lanes == 16);
-}
+bool lanes_ok(int box, int lanes) { return lanes_built(box, lanes); }
 
 // Lanes per ROI for a large batch: the fewest lanes whose kernel still runs
 // two or more waves per SIMD without spilling (the batch autotuner measures).

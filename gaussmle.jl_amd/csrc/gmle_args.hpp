@@ -19,6 +19,14 @@ __host__ __device__ constexpr int n_params(int model) {
   return model == XYNB ? 4 : model == XYNBS ? 5 : model == XYNBSXSY ? 6 : 5;
 }
 
+// The fit kernels built: every box size kMinBox..kMaxBox with 2, 4, 8 or 16
+// lanes per ROI, and with 1 lane up to 12 px, where one lane can still keep a
+// whole ROI resident in VGPRs.
+__host__ __device__ constexpr bool lanes_built(int box, int lanes) {
+  return (box >= kMinBox && box <= kMaxBox) &&
+         ((lanes == 1 && box <= 12) || lanes == 2 || lanes == 4 || lanes == 8 || lanes == 16);
+}
+
 // Largest box size whose kernel with `lanes` lanes per ROI still fits 256
 // VGPRs without spilling, i.e. runs two or more waves per SIMD, with a few
 // registers to spare for the multi-shape kernel's call.  A lane keeps its

@@ -11,11 +11,6 @@
 
 namespace gmle {
 
-constexpr bool lanes_built(int box, int lanes) {
-  return (box >= kMinBox && box <= kMaxBox) &&
-         ((lanes == 1 && box <= 12) || lanes == 2 || lanes == 4 || lanes == 8 || lanes == 16);
-}
-
 // The launcher (Describe = false) or the kernel description (true) of one
 // specialisation, as a function pointer.
 template <int MODEL, int CAM, int PREC, int N, int L, bool Describe>
