@@ -165,43 +165,44 @@ void fill_args(FitArgs& a, int model, const float* psf, const float* lower, cons
 
 namespace {
 
-// Device buffers cached across host-pointer calls (grow only).
+// Device buffers cached across host-pointer calls (grow only), one set per
+// device, each with its own mutex and stream.
 // Slots: 0 data, 1 params, 2 crlb, 3 llr, 4 var_map, 5 corners, 6 crlb64,
 // 7 llr64, 8 models, 9 boxes, 10 offsets, 11 batch workspace.
 constexpr int kCacheSlots = 12;
+constexpr int kMaxDevices = 64;
 struct Cache {
   std::mutex mu;
-  int device = -1;
   void* buf[kCacheSlots] = {};
   size_t cap[kCacheSlots] = {};
   hipStream_t stream = nullptr;
 };
-Cache g_cache;
+Cache g_caches[kMaxDevices];
 
-hipError_t ensure(int slot, size_t bytes) {
-  if (g_cache.cap[slot] >= bytes) return hipSuccess;
-  if (g_cache.buf[slot]) (void)hipFree(g_cache.buf[slot]);
-  g_cache.buf[slot] = nullptr;
-  g_cache.cap[slot] = 0;
-  hipError_t e = hipMalloc(&g_cache.buf[slot], bytes);
-  if (e == hipSuccess) g_cache.cap[slot] = bytes;
+hipError_t ensure(Cache& c, int slot, size_t bytes) {
+  if (c.cap[slot] >= bytes) return hipSuccess;
+  if (c.buf[slot]) (void)hipFree(c.buf[slot]);
+  c.buf[slot] = nullptr;
+  c.cap[slot] = 0;
+  hipError_t e = hipMalloc(&c.buf[slot], bytes);
+  if (e == hipSuccess) c.cap[slot] = bytes;
   return e;
 }
 
-// Points the cache at the current device (caller holds g_cache.mu).
-int cache_on_current_device() {
+// The current device's cache.
+int current_cache(Cache** out) {
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
-  if (g_cache.device != dev) {
-    // Buffers belong to another device: drop them (not freed there, since that
-    // device may not be current) and start over on this one.
-    for (int s = 0; s < kCacheSlots; ++s) { g_cache.buf[s] = nullptr; g_cache.cap[s] = 0; }
-    g_cache.stream = nullptr;
-    g_cache.device = dev;
-  }
-  if (!g_cache.stream) {
-    e = hipStreamCreateWithFlags(&g_cache.stream, hipStreamNonBlocking);
+  if (dev < 0 || dev >= kMaxDevices) return fail(GMLE_ERR_UNSUPPORTED, "device %d beyond the cache table", dev);
+  *out = &g_caches[dev];
+  return GMLE_OK;
+}
+
+// The cache's stream, created on first use (caller holds c.mu).
+int cache_stream(Cache& c) {
+  if (!c.stream) {
+    hipError_t e = hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking);
     if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
   }
   return GMLE_OK;
@@ -211,7 +212,6 @@ int cache_on_current_device() {
 // Mixed-shape batches.
 // ---------------------------------------------------------------------------
 constexpr int kMaxStreams = 4;
-constexpr int kMaxDevices = 64;
 
 // Extra streams (default priority) and fork / join events per device; the
 // caller's stream is the first of the set.
@@ -566,31 +566,33 @@ int gmle_fit(int model, const float* psf, int box, int64_t n_rois, const float* 
   LaunchFn fn = find_launcher(model, camera, box, lanes, precision);
   if (!fn) return fail(GMLE_ERR_UNSUPPORTED, "no kernel for model %d box %d lanes %d", model, box, lanes);
 
-  std::lock_guard<std::mutex> lock(g_cache.mu);
-  rc = cache_on_current_device();
-  if (rc != GMLE_OK) return rc;
+  Cache* cache = nullptr;
+  if ((rc = current_cache(&cache)) != GMLE_OK) return rc;
+  Cache& C = *cache;
+  std::lock_guard<std::mutex> lock(C.mu);
+  if ((rc = cache_stream(C)) != GMLE_OK) return rc;
   hipError_t e = hipSuccess;
   const int P = n_params(model);
   const size_t nvar = camera == SCMOS ? (size_t)ny * (size_t)nx : 0;
   const bool want64 = precision == PREC_MIXED && (crlb64 || llr64);
-  if ((e = ensure(0, npix * sizeof(float))) != hipSuccess) return hip_fail(e, "hipMalloc data");
-  if ((e = ensure(1, (size_t)n_rois * P * sizeof(float))) != hipSuccess) return hip_fail(e, "hipMalloc params");
-  if ((e = ensure(2, (size_t)n_rois * P * sizeof(float))) != hipSuccess) return hip_fail(e, "hipMalloc crlb");
-  if ((e = ensure(3, (size_t)n_rois * sizeof(float))) != hipSuccess) return hip_fail(e, "hipMalloc llr");
+  if ((e = ensure(C, 0, npix * sizeof(float))) != hipSuccess) return hip_fail(e, "hipMalloc data");
+  if ((e = ensure(C, 1, (size_t)n_rois * P * sizeof(float))) != hipSuccess) return hip_fail(e, "hipMalloc params");
+  if ((e = ensure(C, 2, (size_t)n_rois * P * sizeof(float))) != hipSuccess) return hip_fail(e, "hipMalloc crlb");
+  if ((e = ensure(C, 3, (size_t)n_rois * sizeof(float))) != hipSuccess) return hip_fail(e, "hipMalloc llr");
   if (camera == SCMOS) {
-    if ((e = ensure(4, nvar * sizeof(float))) != hipSuccess) return hip_fail(e, "hipMalloc var_map");
-    if ((e = ensure(5, (size_t)n_rois * 2 * sizeof(int32_t))) != hipSuccess) return hip_fail(e, "hipMalloc corners");
+    if ((e = ensure(C, 4, nvar * sizeof(float))) != hipSuccess) return hip_fail(e, "hipMalloc var_map");
+    if ((e = ensure(C, 5, (size_t)n_rois * 2 * sizeof(int32_t))) != hipSuccess) return hip_fail(e, "hipMalloc corners");
   }
   if (want64) {
-    if ((e = ensure(6, (size_t)n_rois * P * sizeof(double))) != hipSuccess) return hip_fail(e, "hipMalloc crlb64");
-    if ((e = ensure(7, (size_t)n_rois * sizeof(double))) != hipSuccess) return hip_fail(e, "hipMalloc llr64");
+    if ((e = ensure(C, 6, (size_t)n_rois * P * sizeof(double))) != hipSuccess) return hip_fail(e, "hipMalloc crlb64");
+    if ((e = ensure(C, 7, (size_t)n_rois * sizeof(double))) != hipSuccess) return hip_fail(e, "hipMalloc llr64");
   }
-  hipStream_t s = g_cache.stream;
-  float* d_data = (float*)g_cache.buf[0];
-  int32_t* d_corners = (int32_t*)g_cache.buf[5];
+  hipStream_t s = C.stream;
+  float* d_data = (float*)C.buf[0];
+  int32_t* d_corners = (int32_t*)C.buf[5];
   e = hipMemcpyAsync(d_data, data, npix * sizeof(float), hipMemcpyHostToDevice, s);
   if (e == hipSuccess && camera == SCMOS) {
-    e = hipMemcpyAsync(g_cache.buf[4], var_map, nvar * sizeof(float), hipMemcpyHostToDevice, s);
+    e = hipMemcpyAsync(C.buf[4], var_map, nvar * sizeof(float), hipMemcpyHostToDevice, s);
     if (e == hipSuccess)
       e = hipMemcpyAsync(d_corners, x_corners, (size_t)n_rois * sizeof(int32_t), hipMemcpyHostToDevice, s);
     if (e == hipSuccess)
@@ -600,14 +602,14 @@ int gmle_fit(int model, const float* psf, int box, int64_t n_rois, const float* 
   FitArgs a;
   fill_args(a, model, psf, lower, upper, max_step, n_rois, ny, nx, iterations);
   a.data = d_data;
-  a.var_map = camera == SCMOS ? (const float*)g_cache.buf[4] : nullptr;
+  a.var_map = camera == SCMOS ? (const float*)C.buf[4] : nullptr;
   a.x_corners = camera == SCMOS ? d_corners : nullptr;
   a.y_corners = camera == SCMOS ? d_corners + n_rois : nullptr;
-  a.params = (float*)g_cache.buf[1];
-  a.crlb = (float*)g_cache.buf[2];
-  a.llr = (float*)g_cache.buf[3];
-  a.crlb64 = want64 ? (double*)g_cache.buf[6] : nullptr;
-  a.llr64 = want64 ? (double*)g_cache.buf[7] : nullptr;
+  a.params = (float*)C.buf[1];
+  a.crlb = (float*)C.buf[2];
+  a.llr = (float*)C.buf[3];
+  a.crlb64 = want64 ? (double*)C.buf[6] : nullptr;
+  a.llr64 = want64 ? (double*)C.buf[7] : nullptr;
   e = fn(a, s);
   if (e != hipSuccess) return hip_fail(e, "kernel launch");
   e = hipMemcpyAsync(params, a.params, (size_t)n_rois * P * sizeof(float), hipMemcpyDeviceToHost, s);
@@ -710,54 +712,56 @@ int gmle_fit_batch(int64_t n_rois, const int8_t* models, const int8_t* boxes, co
   if (rc != GMLE_OK) return rc;
   if (gmle_device_count() <= 0) return fail(GMLE_ERR_NODEVICE, "no HIP device available");
 
-  std::lock_guard<std::mutex> lock(g_cache.mu);
-  rc = cache_on_current_device();
-  if (rc != GMLE_OK) return rc;
+  Cache* cache = nullptr;
+  if ((rc = current_cache(&cache)) != GMLE_OK) return rc;
+  Cache& C = *cache;
+  std::lock_guard<std::mutex> lock(C.mu);
+  if ((rc = cache_stream(C)) != GMLE_OK) return rc;
   const size_t n = (size_t)n_rois;
   const size_t nvar = camera == SCMOS ? (size_t)ny * (size_t)nx : 0;
   const bool want64 = precision == PREC_MIXED && (crlb64 || llr64);
   hipError_t e = hipSuccess;
-  if ((e = ensure(0, (size_t)npix * sizeof(float))) != hipSuccess) return hip_fail(e, "hipMalloc data");
-  if ((e = ensure(1, n * kBatchStride * sizeof(float))) != hipSuccess) return hip_fail(e, "hipMalloc params");
-  if ((e = ensure(2, n * kBatchStride * sizeof(float))) != hipSuccess) return hip_fail(e, "hipMalloc crlb");
-  if ((e = ensure(3, n * sizeof(float))) != hipSuccess) return hip_fail(e, "hipMalloc llr");
+  if ((e = ensure(C, 0, (size_t)npix * sizeof(float))) != hipSuccess) return hip_fail(e, "hipMalloc data");
+  if ((e = ensure(C, 1, n * kBatchStride * sizeof(float))) != hipSuccess) return hip_fail(e, "hipMalloc params");
+  if ((e = ensure(C, 2, n * kBatchStride * sizeof(float))) != hipSuccess) return hip_fail(e, "hipMalloc crlb");
+  if ((e = ensure(C, 3, n * sizeof(float))) != hipSuccess) return hip_fail(e, "hipMalloc llr");
   if (camera == SCMOS) {
-    if ((e = ensure(4, nvar * sizeof(float))) != hipSuccess) return hip_fail(e, "hipMalloc var_map");
-    if ((e = ensure(5, n * 2 * sizeof(int32_t))) != hipSuccess) return hip_fail(e, "hipMalloc corners");
+    if ((e = ensure(C, 4, nvar * sizeof(float))) != hipSuccess) return hip_fail(e, "hipMalloc var_map");
+    if ((e = ensure(C, 5, n * 2 * sizeof(int32_t))) != hipSuccess) return hip_fail(e, "hipMalloc corners");
   }
   if (want64) {
-    if ((e = ensure(6, n * kBatchStride * sizeof(double))) != hipSuccess) return hip_fail(e, "hipMalloc crlb64");
-    if ((e = ensure(7, n * sizeof(double))) != hipSuccess) return hip_fail(e, "hipMalloc llr64");
+    if ((e = ensure(C, 6, n * kBatchStride * sizeof(double))) != hipSuccess) return hip_fail(e, "hipMalloc crlb64");
+    if ((e = ensure(C, 7, n * sizeof(double))) != hipSuccess) return hip_fail(e, "hipMalloc llr64");
   }
-  if ((e = ensure(8, n)) != hipSuccess) return hip_fail(e, "hipMalloc models");
-  if ((e = ensure(9, n)) != hipSuccess) return hip_fail(e, "hipMalloc boxes");
-  if ((e = ensure(10, n * sizeof(int64_t))) != hipSuccess) return hip_fail(e, "hipMalloc offsets");
-  if ((e = ensure(11, batch_workspace(n_rois))) != hipSuccess) return hip_fail(e, "hipMalloc workspace");
-  hipStream_t s = g_cache.stream;
-  int32_t* d_corners = (int32_t*)g_cache.buf[5];
-  e = hipMemcpyAsync(g_cache.buf[0], data, (size_t)npix * sizeof(float), hipMemcpyHostToDevice, s);
-  if (e == hipSuccess) e = hipMemcpyAsync(g_cache.buf[8], models, n, hipMemcpyHostToDevice, s);
-  if (e == hipSuccess) e = hipMemcpyAsync(g_cache.buf[9], boxes, n, hipMemcpyHostToDevice, s);
-  if (e == hipSuccess) e = hipMemcpyAsync(g_cache.buf[10], offsets.data(), n * sizeof(int64_t), hipMemcpyHostToDevice, s);
+  if ((e = ensure(C, 8, n)) != hipSuccess) return hip_fail(e, "hipMalloc models");
+  if ((e = ensure(C, 9, n)) != hipSuccess) return hip_fail(e, "hipMalloc boxes");
+  if ((e = ensure(C, 10, n * sizeof(int64_t))) != hipSuccess) return hip_fail(e, "hipMalloc offsets");
+  if ((e = ensure(C, 11, batch_workspace(n_rois))) != hipSuccess) return hip_fail(e, "hipMalloc workspace");
+  hipStream_t s = C.stream;
+  int32_t* d_corners = (int32_t*)C.buf[5];
+  e = hipMemcpyAsync(C.buf[0], data, (size_t)npix * sizeof(float), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(C.buf[8], models, n, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(C.buf[9], boxes, n, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(C.buf[10], offsets.data(), n * sizeof(int64_t), hipMemcpyHostToDevice, s);
   if (e == hipSuccess && camera == SCMOS) {
-    e = hipMemcpyAsync(g_cache.buf[4], var_map, nvar * sizeof(float), hipMemcpyHostToDevice, s);
+    e = hipMemcpyAsync(C.buf[4], var_map, nvar * sizeof(float), hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(d_corners, x_corners, n * sizeof(int32_t), hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(d_corners + n, y_corners, n * sizeof(int32_t), hipMemcpyHostToDevice, s);
   }
   if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync (upload)");
-  b.models = (const int8_t*)g_cache.buf[8];
-  b.boxes = (const int8_t*)g_cache.buf[9];
-  b.offsets = (const int64_t*)g_cache.buf[10];
-  b.data = (const float*)g_cache.buf[0];
-  b.var_map = camera == SCMOS ? (const float*)g_cache.buf[4] : nullptr;
+  b.models = (const int8_t*)C.buf[8];
+  b.boxes = (const int8_t*)C.buf[9];
+  b.offsets = (const int64_t*)C.buf[10];
+  b.data = (const float*)C.buf[0];
+  b.var_map = camera == SCMOS ? (const float*)C.buf[4] : nullptr;
   b.xc = camera == SCMOS ? d_corners : nullptr;
   b.yc = camera == SCMOS ? d_corners + n : nullptr;
-  b.params = (float*)g_cache.buf[1];
-  b.crlb = (float*)g_cache.buf[2];
-  b.llr = (float*)g_cache.buf[3];
-  b.crlb64 = want64 ? (double*)g_cache.buf[6] : nullptr;
-  b.llr64 = want64 ? (double*)g_cache.buf[7] : nullptr;
-  b.workspace = g_cache.buf[11];
+  b.params = (float*)C.buf[1];
+  b.crlb = (float*)C.buf[2];
+  b.llr = (float*)C.buf[3];
+  b.crlb64 = want64 ? (double*)C.buf[6] : nullptr;
+  b.llr64 = want64 ? (double*)C.buf[7] : nullptr;
+  b.workspace = C.buf[11];
   rc = enqueue_batch(b, s);
   if (rc != GMLE_OK) return rc;
   e = hipMemcpyAsync(params, b.params, n * kBatchStride * sizeof(float), hipMemcpyDeviceToHost, s);
