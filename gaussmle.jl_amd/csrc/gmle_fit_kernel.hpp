@@ -195,16 +195,16 @@ __device__ __forceinline__ Pix<T> pix_from_edges(const AxisK<T>& k, const Edge<T
 // can still round a rare one differently, so a kernel and its replicated twin
 // may differ in the last bit of an ROI's result.
 // ---------------------------------------------------------------------------
-// Where sharing is used: from 4 lanes up, except in the largest 4-lane tile
-// of each model and camera that still fits 256 VGPRs (two waves per SIMD,
-// max_box_within_256): sharing keeps a few more values live, which would push
-// exactly those tiles past it (gfx950 register counts).  2-lane kernels keep
-// the replicated evaluation: there sharing would only halve the edge work,
-// and the multi-shape kernel, which holds every 1-, 2- and 4-lane tile in one
-// function, would no longer fit two waves per SIMD.
+// Where sharing is used (gfx950 register counts): sharing keeps a few more
+// values live, so the tiles at the top of each two-wave range keep the
+// replicated evaluation: with 4 lanes the largest box that fits 256 VGPRs
+// (max_box_within_256), with 2 lanes the two largest, since the multi-shape
+// kernel, which calls every 1-, 2- and 4-lane tile, has to stay at two waves
+// per SIMD too.  From 8 lanes up every tile shares.
 template <int MODEL, int CAM, int N, int L> constexpr bool share_rows() {
-  if (L < 4) return false;
+  if (L == 1) return false;
   if (L >= 8) return true;
+  if (L == 2) return N + 1 < max_box_within_256(MODEL, CAM, L);
   return N != max_box_within_256(MODEL, CAM, L);
 }
 
