@@ -569,8 +569,10 @@ def main(argv=None) -> int:
         trans = mix["valu_trans_f32"] * n / avg_kernel_s
         roofline["trans_per_fit"] = mix["valu_trans_f32"]
         roofline["trans_util"] = round(trans / (VALU_LANE_OPS_PER_S / 4), 4)
+        # Instructions per fit from the profile; the issue-slot share comes
+        # from measured GPU cycles there (GRBM_GUI_ACTIVE), not from a clock
+        # assumed here.
         roofline["valu_insts_per_fit"] = mix["valu"]
-        roofline["valu_issue_util"] = round(mix["valu"] * n / avg_kernel_s / VALU_LANE_OPS_PER_S, 4)
 
     cpu = None
     if not args.no_cpu_baseline:
