@@ -616,7 +616,37 @@ template <int N, bool SC> struct RegSrc {
   const float* vr;
   __device__ __forceinline__ float d(int cc, int i) const { return dv[cc * N + i]; }
   __device__ __forceinline__ float var(int cc, int i) const { return SC ? vr[cc * N + i] : 0.f; }
+  __device__ __forceinline__ void column(int cc, float* dcol, float* vcol) const {
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      dcol[i] = dv[cc * N + i];
+      if constexpr (SC) vcol[i] = vr[cc * N + i];
+    }
+  }
 };
+// n consecutive floats from p (4-byte aligned) in 16- and 8-byte loads: gfx950
+// global loads need only dword alignment, and one lane's run of a column then
+// takes ceil(n / 4) load instructions instead of n.
+template <int n> __device__ __forceinline__ void load_run(const float* p, float* out) {
+#pragma unroll
+  for (int i = 0; i + 4 <= n; i += 4) {
+    float4 q;
+    __builtin_memcpy(&q, p + i, sizeof(q));
+    out[i] = q.x;
+    out[i + 1] = q.y;
+    out[i + 2] = q.z;
+    out[i + 3] = q.w;
+  }
+  constexpr int b = n - n % 4;
+  if constexpr (n % 4 >= 2) {
+    float2 q;
+    __builtin_memcpy(&q, p + b, sizeof(q));
+    out[b] = q.x;
+    out[b + 1] = q.y;
+  }
+  if constexpr (n % 2 == 1) out[n - 1] = p[n - 1];
+}
+
 template <int N, bool SC> struct GlobalSrc {
   static constexpr bool kInRegisters = false;
   const float* data = nullptr;   // the ROI, column-major
@@ -624,6 +654,11 @@ template <int N, bool SC> struct GlobalSrc {
   int ny = 0;
   __device__ __forceinline__ float d(int cc, int i) const { return data[cc * N + i]; }
   __device__ __forceinline__ float var_at(int cc, int i) const { return var[(int64_t)cc * ny + i]; }
+  // column cc's pixels and variances (both runs are contiguous)
+  __device__ __forceinline__ void column(int cc, float* dcol, float* vcol) const {
+    load_run<N>(data + cc * N, dcol);
+    if constexpr (SC) load_run<N>(var + (int64_t)cc * ny, vcol);
+  }
 };
 template <int N, bool SC> __device__ __forceinline__ float src_var(const RegSrc<N, SC>& s, int cc, int i) {
   return s.var(cc, i);
@@ -668,10 +703,12 @@ __device__ __forceinline__ FinalOut<TF, MT<MODEL>::P> final_pass(const float* th
     const bool cvalid = (L == 1) || (c0 + cc < N);
     TF ax = TF(0);
     if constexpr (SIG) ax = (MODEL == ASTIG) ? wf.dsx * fx.S : fx.S;
+    float dcol[N], vcol[CAM == SCMOS ? N : 1];
+    src.column(cc, dcol, vcol);
 #pragma unroll
     for (int i = 0; i < N; ++i) {
       const TF py = fy.phi[i];
-      const TF d = (TF)src.d(cc, i);
+      const TF d = (TF)dcol[i];
       const TF mu = bgf + nphf * px * py;
       TF du[P];
       du[Tr::IX] = fx.D * py;
@@ -699,7 +736,7 @@ __device__ __forceinline__ FinalOut<TF, MT<MODEL>::P> final_pass(const float* th
         }
       } else {
         // camera_models.jl:73-85
-        v = mu + (TF)src_var(src, cc, i);
+        v = mu + (TF)vcol[i];
         const TF res = d - mu;
         lp = v > TF(0) ? TF(-0.5) * (res * res / v) : TF(0);
       }
