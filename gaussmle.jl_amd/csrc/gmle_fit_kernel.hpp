@@ -843,20 +843,29 @@ __device__ __forceinline__ void fit_one(const FitArgs& a, const RoiRef& ref, boo
     cyc = ref.cy;
     inb = (cxc >= 1) && (cyc >= 1) && (cxc <= a.nx - N + 1) && (cyc <= a.ny - N + 1);
   }
+  // Each column is a contiguous run of N floats in the ROI and in the variance
+  // map: read in 16-byte loads (load_run), since a wave's lanes are ROIs apart
+  // and every load instruction touches one cache line per lane.
 #pragma unroll
   for (int cc = 0; cc < CB; ++cc) {
     const int col = c0 + cc;
     const bool valid = (L == 1) || (col < N);
+    if (valid) {
+      load_run<N>(src + col * N, dv + cc * N);
+    } else {
 #pragma unroll
-    for (int i = 0; i < N; ++i) {
-      const float v = valid ? src[col * N + i] : 0.f;
-      dv[cc * N + i] = v;
-      dmax_l = fmaxf(dmax_l, v);
-      if constexpr (CAM == SCMOS) {
-        const int64_t off = (int64_t)(cyc - 1 + i) + (int64_t)a.ny * (int64_t)(cxc - 1 + col);
-        vr[cc * N + i] = (valid && inb) ? a.var_map[off] : 0.f;
+      for (int i = 0; i < N; ++i) dv[cc * N + i] = 0.f;
+    }
+    if constexpr (CAM == SCMOS) {
+      if (valid && inb) {
+        load_run<N>(a.var_map + (int64_t)(cyc - 1) + (int64_t)a.ny * (int64_t)(cxc - 1 + col), vr + cc * N);
+      } else {
+#pragma unroll
+        for (int i = 0; i < N; ++i) vr[cc * N + i] = 0.f;
       }
     }
+#pragma unroll
+    for (int i = 0; i < N; ++i) dmax_l = fmaxf(dmax_l, dv[cc * N + i]);
   }
   const float dmax = group_max<L>(dmax_l);
 
