@@ -1092,9 +1092,12 @@ __global__ __launch_bounds__(kWave) void fit_kernel(FitArgs a) {
 
 // Float64 final pass at the estimates the iteration kernel wrote to params,
 // one lane per ROI; the ROI is re-read from HBM (once) instead of being held
-// in VGPRs.
+// in VGPRs.  Two waves per SIMD are requested: from 12 px up the compiler then
+// spills a few dozen values to scratch (astig 15 px: 284 -> 256 VGPRs, 29
+// spilled), which costs less than running one wave per SIMD (gfx950: the
+// astigmatic 15 px pass 18 % faster).
 template <int MODEL, int CAM, int N>
-__global__ __launch_bounds__(kWave) void final_pass_kernel(FitArgs a) {
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(2))) void final_pass_kernel(FitArgs a) {
   constexpr int P = MT<MODEL>::P;
   if (a.n_rois <= 0 || (a.order && __builtin_amdgcn_readfirstlane(a.order[a.first]) < 0)) return;
   RoiRef ref;
