@@ -726,10 +726,13 @@ __device__ __forceinline__ FinalOut<TF, MT<MODEL>::P> final_pass(const float* th
       }
       TF v = mu;
       TF lp;
+      TF rmu = TF(0);   // 1/mu, Float64 ideal camera only
       if constexpr (CAM == IDEAL) {
         // camera_models.jl:60-71; in Float64 as d log(mu/d) + d - mu (one log)
         if constexpr (sizeof(TF) == 8) {
-          lp = (mu > TF(0) && d > TF(0)) ? d * log(mu / d) + (d - mu) : (mu > TF(0) && d == TF(0)) ? -mu : TF(0);
+          // d log(mu/d) = -d log(d (1/mu)); 1/mu is also the Fisher weight below
+          rmu = mu > TF(0) ? TF(1) / mu : TF(0);
+          lp = (mu > TF(0) && d > TF(0)) ? -d * log(d * rmu) + (d - mu) : (mu > TF(0) && d == TF(0)) ? -mu : TF(0);
         } else {
           lp = (mu > TF(0) && d > TF(0)) ? d * logf(mu) - mu - (d * logf(d) - d)
                                         : (mu > TF(0) && d == TF(0)) ? -mu : TF(0);
@@ -742,7 +745,7 @@ __device__ __forceinline__ FinalOut<TF, MT<MODEL>::P> final_pass(const float* th
       }
       if (cvalid) llr += lp;
       if (cvalid && mu > TF(0)) {
-        const TF wv = TF(1) / v;
+        const TF wv = (CAM == IDEAL && sizeof(TF) == 8) ? rmu : TF(1) / v;
         int q = 0;
 #pragma unroll
         for (int k = 0; k < P; ++k) {
