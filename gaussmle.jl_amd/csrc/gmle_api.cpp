@@ -214,8 +214,11 @@ int cache_stream(Cache& c) {
 constexpr int kMaxStreams = 4;
 
 // Extra streams (default priority) and fork / join events per device; the
-// caller's stream is the first of the set.
+// caller's stream is the first of the set.  `init` guards the one-time
+// creation (handles are made once and never replaced); `mu` is held by a
+// call for as long as it uses the streams and events.
 struct StreamSet {
+  std::mutex init;
   std::mutex mu;
   bool ready = false;
   hipStream_t s[kMaxStreams - 1] = {};
@@ -230,15 +233,39 @@ int stream_set(StreamSet** out) {
   if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
   if (dev < 0 || dev >= kMaxDevices) return fail(GMLE_ERR_UNSUPPORTED, "device %d beyond the stream table", dev);
   StreamSet& ss = g_streams[dev];
+  std::lock_guard<std::mutex> guard(ss.init);
   if (!ss.ready) {
+    // Created into locals and published only when all exist, so a failure
+    // leaves the set empty for the next call to retry.
+    hipStream_t s[kMaxStreams - 1] = {};
+    hipEvent_t join[kMaxStreams - 1] = {};
+    hipEvent_t fork = nullptr;
+    auto undo = [&]() {
+      for (int k = 0; k < kMaxStreams - 1; ++k) {
+        if (s[k]) (void)hipStreamDestroy(s[k]);
+        if (join[k]) (void)hipEventDestroy(join[k]);
+      }
+      if (fork) (void)hipEventDestroy(fork);
+    };
     for (int k = 0; k < kMaxStreams - 1; ++k) {
-      if ((e = hipStreamCreateWithFlags(&ss.s[k], hipStreamNonBlocking)) != hipSuccess)
+      if ((e = hipStreamCreateWithFlags(&s[k], hipStreamNonBlocking)) != hipSuccess) {
+        undo();
         return hip_fail(e, "hipStreamCreate");
-      if ((e = hipEventCreateWithFlags(&ss.join[k], hipEventDisableTiming)) != hipSuccess)
+      }
+      if ((e = hipEventCreateWithFlags(&join[k], hipEventDisableTiming)) != hipSuccess) {
+        undo();
         return hip_fail(e, "hipEventCreate");
+      }
     }
-    if ((e = hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming)) != hipSuccess)
+    if ((e = hipEventCreateWithFlags(&fork, hipEventDisableTiming)) != hipSuccess) {
+      undo();
       return hip_fail(e, "hipEventCreate");
+    }
+    for (int k = 0; k < kMaxStreams - 1; ++k) {
+      ss.s[k] = s[k];
+      ss.join[k] = join[k];
+    }
+    ss.fork = fork;
     ss.ready = true;
   }
   *out = &ss;

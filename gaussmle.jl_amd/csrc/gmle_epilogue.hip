@@ -32,8 +32,11 @@ namespace {
 // ratio settles within an ulp of 1, and would run it to the iteration cap.)
 constexpr double kTol = 1e-15;
 
+// x <= 0 gives 0 (a chi^2 below zero, from an LLR rounded just above zero,
+// has CDF 0 as in Distributions.jl, whose gamma CDF clamps x at 0); NaN stays NaN.
 __host__ __device__ inline double gamma_p(double a, double x) {
-  if (!(x > 0.0)) return x == 0.0 ? 0.0 : __builtin_nan("");
+  if (x != x) return x;
+  if (!(x > 0.0)) return 0.0;
   if (isinf(x)) return 1.0;
   const double lg = lgamma(a);
   const double lead = exp(-x + a * log(x) - lg);

@@ -153,7 +153,9 @@ int gmle_simulate_device(int model, const float* psf, int box, int64_t n_rois, c
   if (camera == SCMOS && (!(gain != 0.0f) && !d_gain)) return fail(GMLE_ERR_ARG, "gain must be non-zero");
   SimArgs a{};
   a.model = model;
-  if (psf)
+  // XYNB reads one float (sigma), ASTIG eight, the others none, as gmle_fit.
+  if (model == XYNB) a.psf[0] = psf[0];
+  if (model == ASTIG)
     for (int q = 0; q < 8; ++q) a.psf[q] = psf[q];
   a.box = box;
   a.n = n_rois;

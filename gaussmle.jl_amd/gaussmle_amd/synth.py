@@ -213,6 +213,53 @@ def generic(psf_pixels: PSFModel, n: int, box: int, seed: int, iterations: int =
     return Workload(name, psf_pixels, box, data, theta, constraints, iterations, var, xc, yc)
 
 
+def low_background(psf_pixels: PSFModel, n: int, box: int, seed: int, iterations: int = 20,
+                   bg_range=(0.002, 0.04), log_n_range=(200.0, 1.0e5), hot_fraction: float = 0.25,
+                   hot_range=(200.0, 2000.0), bg_lower: float = 1e-4,
+                   sigma_range=(1.0, 1.6)) -> Workload:
+    """Poisson ROIs on which the reference's likelihood branches bind
+    (src/camera_models.jl:23-32): background U(bg_range) photons/px (below the
+    0.01 model threshold in the tails), N log-uniform in log_n_range, and in a
+    hot_fraction of the ROIs one hot pixel away from the spot (+U(hot_range)
+    counts, as a cosmic-ray hit), where data/model exceeds the 1e4 caps.  The
+    background's lower bound is bg_lower (the default constraints' 0.01 would
+    clamp it above the threshold)."""
+    rng = np.random.default_rng(seed)
+    c = (box + 1) / 2
+    x = rng.uniform(c - 1.5, c + 1.5, n)
+    y = rng.uniform(c - 1.5, c + 1.5, n)
+    N = np.exp(rng.uniform(np.log(log_n_range[0]), np.log(log_n_range[1]), n))
+    bg = rng.uniform(*bg_range, n)
+    if isinstance(psf_pixels, GaussianXYNB):
+        theta = np.stack([x, y, N, bg])
+    elif isinstance(psf_pixels, GaussianXYNBS):
+        theta = np.stack([x, y, N, bg, rng.uniform(*sigma_range, n)])
+    elif isinstance(psf_pixels, GaussianXYNBSXSY):
+        theta = np.stack([x, y, N, bg, rng.uniform(*sigma_range, n), rng.uniform(*sigma_range, n)])
+    else:
+        theta = np.stack([x, y, rng.uniform(-300, 300, n), N, bg])
+    data = _poisson_stack(rng, expected_counts(psf_pixels, theta, box))
+    hot = np.flatnonzero(rng.random(n) < hot_fraction)
+    # an interior pixel (the border ring sets the initial background) at least
+    # 3 px from the spot centre, where the model is near the background
+    for k in hot:
+        while True:
+            i, j = rng.integers(1, box - 1, 2)
+            if max(abs(i + 1 - theta[1, k]), abs(j + 1 - theta[0, k])) >= 3:
+                break
+        data[i, j, k] += F32(np.floor(rng.uniform(*hot_range)))
+    if isinstance(psf_pixels, AstigmaticXYZNB):
+        cons = astig_constraints()
+    else:
+        cons = default_constraints(psf_pixels, 11)
+    ibg = 4 if isinstance(psf_pixels, AstigmaticXYZNB) else 3
+    lower = np.array(cons.lower, dtype=F32)
+    lower[ibg] = F32(bg_lower)
+    cons = ParameterConstraints(lower, np.array(cons.upper, dtype=F32), np.array(cons.max_step, dtype=F32))
+    name = f"{type(psf_pixels).__name__}_lowbg_{box}"
+    return Workload(name, psf_pixels, box, data, theta, cons, iterations)
+
+
 def spot_frames(ny: int, nx: int, nf: int, spots_per_frame: int, seed: int = 5, bg: float = 10.0,
                 photons=(500.0, 3000.0), sigma: float = 1.3, edge: int = 4, window: int = 8):
     """Camera frames (ny, nx, nf) Float32, column-major, in photons: a flat

@@ -28,9 +28,48 @@ function last_error()
     return unsafe_string(pointer(buf))
 end
 
-# Model id of each GaussMLE PSF type, and its 8-float pixel-unit parameter block.
-model_id(::Any) = error("unsupported PSF model")
-psf_block(psf) = zeros(Float32, 8)
+# Model id of each GaussMLE PSF type (enum gmle_model), and its 8-float
+# pixel-unit parameter block (the `psf` argument of gmle_fit): GaussianXYNB
+# [σ, 0, ...], AstigmaticXYZNB [σx₀, σy₀, Ax, Ay, Bx, By, γ, d], zeros for the
+# fitted-width models.  Pass the PSF after to_pixel_units
+# (src/psf_models.jl:311-331): σ, σx₀, σy₀ in pixels, γ and d as given.
+# The PSF types are GaussMLE.jl's (src/psf_models.jl:39, 64, 91, 141); this file
+# is included inside module GaussMLE after psf_models.jl, so they are the
+# parent module's.  Loaded on its own, only the fallbacks exist.
+model_id(psf) = throw(ArgumentError("unsupported PSF model $(typeof(psf))"))
+psf_block(psf) = throw(ArgumentError("unsupported PSF model $(typeof(psf))"))
+
+const _GMLE = parentmodule(@__MODULE__)
+if isdefined(_GMLE, :GaussianXYNB) && isdefined(_GMLE, :GaussianXYNBS) &&
+   isdefined(_GMLE, :GaussianXYNBSXSY) && isdefined(_GMLE, :AstigmaticXYZNB)
+    model_id(::_GMLE.GaussianXYNB) = Cint(0)
+    model_id(::_GMLE.GaussianXYNBS) = Cint(1)
+    model_id(::_GMLE.GaussianXYNBSXSY) = Cint(2)
+    model_id(::_GMLE.AstigmaticXYZNB) = Cint(3)
+
+    psf_block(p::_GMLE.GaussianXYNB) = Float32[p.σ, 0, 0, 0, 0, 0, 0, 0]
+    psf_block(::_GMLE.GaussianXYNBS) = zeros(Float32, 8)
+    psf_block(::_GMLE.GaussianXYNBSXSY) = zeros(Float32, 8)
+    psf_block(p::_GMLE.AstigmaticXYZNB) = Float32[p.σx₀, p.σy₀, p.Ax, p.Ay, p.Bx, p.By, p.γ, p.d]
+end
+
+"""
+    hip_fit!(results, uncertainties, log_likelihoods, data, psf_pixels, scmos,
+             variance_map, x_corners, y_corners, constraints, iterations; kwargs...)
+
+The reference kernel's own argument list (src/unified_kernel.jl:285-297) with
+the PSF struct and `ParameterConstraints` as GaussMLE.jl holds them: the model
+id and PSF block come from `model_id` / `psf_block`, the constraint vectors
+from the struct's `lower` / `upper` / `max_step` fields (src/constraints.jl:6-10).
+"""
+function hip_fit!(results::Matrix{Float32}, uncertainties::Matrix{Float32}, log_likelihoods::Vector{Float32},
+                  data::Array{Float32,3}, psf_pixels, scmos::Bool, variance_map::Matrix{Float32},
+                  x_corners::Vector{Int32}, y_corners::Vector{Int32}, constraints, iterations::Integer; kwargs...)
+    return hip_fit!(results, uncertainties, log_likelihoods, data, model_id(psf_pixels), psf_block(psf_pixels),
+                    scmos, variance_map, x_corners, y_corners, Vector{Float32}(constraints.lower),
+                    Vector{Float32}(constraints.upper), Vector{Float32}(constraints.max_step), iterations;
+                    kwargs...)
+end
 
 """
     hip_fit!(results, uncertainties, log_likelihoods, data, model, psf_block,

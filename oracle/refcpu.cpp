@@ -186,6 +186,30 @@ int refcpu_fit(int prec, int model, const float* psf, int box, int64_t n_rois, c
   return 0;
 }
 
+// Coverage counters of the Float32 (reference) fit of n_rois Poisson ROIs:
+// branches (n_rois x 4) as fit_roi documents.  Test infrastructure only.
+int refcpu_poisson_branches(int model, const float* psf, int box, int64_t n_rois, const float* data,
+                            const float* lower, const float* upper, const float* max_step, int iterations,
+                            int32_t* branches, int nthreads) {
+  if (box < 3 || box > 32 || n_rois < 0 || model < 0 || model > 3 || !branches) return -1;
+  const int P = n_params(model);
+  const Psf<float> p = make_psf(model, psf);
+  Bounds b;
+  for (int k = 0; k < P; ++k) {
+    b.lower[k] = lower[k]; b.upper[k] = upper[k]; b.max_step[k] = max_step[k];
+  }
+  const int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nt)
+  for (int64_t r = 0; r < n_rois; ++r) {
+    float th[6], cr[6], l;
+    int32_t* br = branches + 4 * r;
+    br[0] = br[1] = br[2] = br[3] = 0;
+    fit_roi<float, float>(data + (size_t)r * box * box, box, p, false, nullptr, 0, 1, 1, b, iterations, th, cr, &l,
+                          br);
+  }
+  return 0;
+}
+
 // The final pass at given estimates theta (n_rois x P, Float64): LLR, Fisher,
 // inversion and CRLB in Float32 (prec 0) or Float64 (prec 1 / 2).
 int refcpu_final_pass(int prec, int model, const float* psf, int box, int64_t n_rois, const float* data,

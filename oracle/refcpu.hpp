@@ -469,10 +469,16 @@ struct Bounds {
   float lower[6], upper[6], max_step[6];
 };
 
+// branches (optional, test coverage only): over all Newton iterations, the
+// Poisson pixel terms that took [0] the model <= 0.01 branch, [1] the cf cap,
+// [2] the df cap (camera_models.jl:23-32), and [3] the iterations in which the
+// GPU kernel's precondition for skipping those branches (bg >= 0.05,
+// 0 <= N <= 1e5, max(data) <= 5000 bg min(bg, 1), finite parameters) fails,
+// i.e. in which the kernel evaluates them.  Not part of the reference.
 template <class TI, class TF>
 inline int fit_roi(const float* roi_f32, int n, const Psf<float>& psf_f32, bool scmos, const float* var,
                    int ny, int cx, int cy, const Bounds& cons, int iterations, TF* theta_out, TF* crlb_out,
-                   TF* llr_out) {
+                   TF* llr_out, int32_t* branches = nullptr) {
   (void)ny;
   const int P = n_params(psf_f32.model);
   const Psf<TI> psf = psf_cast<TI>(psf_f32);
@@ -486,7 +492,18 @@ inline int fit_roi(const float* roi_f32, int n, const Psf<float>& psf_f32, bool 
   TI th[6];
   initialize(roi, n, psf, th);
   TI grad[6], hd[6], dudt[6], d2[6];
+  TI dmax = TI(0);
+  for (int q = 0; q < n * n; ++q) dmax = roi[q] > dmax ? roi[q] : dmax;
+  const int inb = psf.model == ASTIG ? 3 : 2, ibg = psf.model == ASTIG ? 4 : 3;
   for (int it = 0; it < iterations; ++it) {
+    if (branches && !scmos) {
+      bool fin = true;
+      for (int k = 0; k < P; ++k) fin = fin && std::fabs(th[k]) < TI(3.0e38);
+      const TI bg = th[ibg], nph = th[inb];
+      const bool skip = fin && bg >= TI(0.05) && nph >= TI(0) && nph <= TI(1.0e5) &&
+                        dmax <= TI(5000) * bg * (bg < TI(1) ? bg : TI(1));
+      if (!skip) branches[3] += 1;
+    }
     for (int k = 0; k < P; ++k) { grad[k] = TI(0); hd[k] = TI(0); }
     for (int j = 1; j <= n; ++j)
       for (int i = 1; i <= n; ++i) {
@@ -495,6 +512,14 @@ inline int fit_roi(const float* roi_f32, int n, const Psf<float>& psf_f32, bool 
         TI cf, df;
         if (scmos) likelihood_terms_scmos(data, model, (TI)var_at(i, j), cf, df);
         else likelihood_terms_poisson(data, model, cf, df);
+        if (branches && !scmos) {
+          if (!(model > TI(0.01))) {
+            branches[0] += 1;
+          } else {
+            if (data / model - TI(1) > TI(1e4)) branches[1] += 1;
+            if (data / (model * model) > TI(1e4)) branches[2] += 1;
+          }
+        }
         for (int k = 0; k < P; ++k) {
           grad[k] += dudt[k] * cf;
           hd[k] += d2[k] * cf - dudt[k] * dudt[k] * df;

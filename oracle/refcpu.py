@@ -61,6 +61,9 @@ def load() -> C.CDLL:
         lib.refcpu_fit.restype = C.c_int
         lib.refcpu_fit.argtypes = [C.c_int, C.c_int, _F, C.c_int, C.c_int64, _F, C.c_int, _F, C.c_int, C.c_int,
                                    _I, _I, _F, _F, _F, C.c_int, _D, _D, _D, _I, C.c_int]
+        lib.refcpu_poisson_branches.restype = C.c_int
+        lib.refcpu_poisson_branches.argtypes = [C.c_int, _F, C.c_int, C.c_int64, _F, _F, _F, _F, C.c_int, _I,
+                                                C.c_int]
         lib.refcpu_final_pass.restype = C.c_int
         lib.refcpu_final_pass.argtypes = [C.c_int, C.c_int, _F, C.c_int, C.c_int64, _F, C.c_int, _F, C.c_int,
                                           C.c_int, _I, _I, _D, _D, _D, C.c_int]
@@ -186,6 +189,24 @@ def fit_workload(w, prec: int = FP32, nthreads: int = 0, n: int | None = None):
     c = ww.constraints
     return fit(ww.psf_pixels.model_id, ww.psf_pixels.psf_array(), ww.box, ww.flat_data(), c.lower, c.upper,
                c.max_step, ww.iterations, prec, ww.var_map, ww.x_corners, ww.y_corners, nthreads)
+
+
+def poisson_branches(w, nthreads: int = 0) -> np.ndarray:
+    """Coverage counters of the Float32 fit of a Poisson synth.Workload, (n, 4):
+    pixel terms below the 0.01 threshold, cf caps, df caps (summed over the
+    Newton iterations), and iterations in which the GPU kernel evaluates these
+    branches (its fast-path precondition fails)."""
+    lib = load()
+    c = w.constraints
+    d = np.ascontiguousarray(w.flat_data(), dtype=np.float32)
+    out = np.zeros((w.n, 4), dtype=np.int32)
+    psf = _psf(w.psf_pixels.model_id, w.psf_pixels.psf_array())
+    rc = lib.refcpu_poisson_branches(w.psf_pixels.model_id, psf.ctypes.data_as(_F), w.box, w.n,
+                                     d.ctypes.data_as(_F), _f(c.lower), _f(c.upper), _f(c.max_step), w.iterations,
+                                     out.ctypes.data_as(_I), nthreads)
+    if rc != 0:
+        raise ValueError("refcpu_poisson_branches: bad arguments")
+    return out
 
 
 def final_pass(w, theta: np.ndarray, prec: int = FP64, nthreads: int = 0):

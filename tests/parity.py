@@ -25,8 +25,9 @@ Float32 ulp, so no Float32 implementation can be held to them.  An ROI is
 well-posed when
   (1) the Float32 oracle meets every tolerance against the Float64 oracle, and
   (2) in mixed precision, when given, the oracle's own mixed mode meets the
-      LLR tolerance against the Float64 oracle (the LLR is evaluated at the
-      Float32 iterations' estimate);
+      LLR and CRLB tolerances against the Float64 oracle (both are evaluated
+      in Float64 at the Float32 iterations' estimate; the Float32 CRLB of (1)
+      is not what this mode computes);
   (3) when given, the Float64 oracle on the data perturbed by a few ulp per
       pixel (perturb_ulp: 4 ulp, about the rounding of a Float32 sum over a
       hundred pixel terms, sqrt(121)/2 ulp) meets every tolerance against the
@@ -119,6 +120,14 @@ def compare(model_id: int, gpu: dict, ref32: dict, ref64: dict, precision: str, 
         if refmix is not None:
             eref["llr"] = np.abs(np.asarray(refmix["llr"], np.float64) - ref64["llr"]) / np.maximum(
                 np.abs(ref64["llr"]), 10.0)
+            # likewise the CRLB: a Float64 inversion at the Float32 estimate,
+            # so the oracle's mixed mode decides (a Float32 inversion of a
+            # badly scaled Fisher matrix, e.g. a background near 0.01, would
+            # exclude ROIs the mixed mode fits well)
+            cm = _rel(np.asarray(refmix["crlb"], np.float64), ref64["crlb"])
+            cm[np.isinf(refmix["crlb"]) & np.isinf(ref64["crlb"])] = 0.0
+            for k in range(cm.shape[1]):
+                eref[f"crlb{k}"] = cm[:, k]
         else:
             eref["llr"] = np.zeros_like(eref["llr"])
     well = ~_bad(eref)

@@ -83,3 +83,29 @@ def test_concurrent_device_fits_match_serial():
     for i, (a, b) in enumerate(zip(par, serial)):
         for x, y, name in zip(a, b, ("params", "crlb", "llr")):
             np.testing.assert_array_equal(x, y, err_msg=f"workload {i} {name}")
+
+
+def test_concurrent_batch_device_fits_with_streams_match_serial():
+    """gmle_fit_batch_device with streams = 4 from four threads at once: the
+    shared fork / join streams are created once (under their own lock) and
+    each call holds them for its whole enqueue, so every result equals the
+    serial call bit for bit."""
+    import torch
+    from gaussmle_amd import batch as gb
+    mbs = [gb.mixed_workload(3000 + 500 * i, seed=310 + i, boxes=(7, 9, 11), box_weights=(1, 1, 1))
+           for i in range(4)]
+    ups = [gb.upload_batch(mb) for mb in mbs]
+    torch.cuda.synchronize()
+
+    def batch_fit(i):
+        s = torch.cuda.Stream()
+        out = gb.fit_batch_device(mbs[i], ups[i], "fp32", None, 4, stream=s)
+        s.synchronize()
+        return out["params"].cpu().numpy(), out["crlb"].cpu().numpy(), out["llr"].cpu().numpy()
+
+    serial = [batch_fit(i) for i in range(4)]
+    for rnd in range(3):
+        par = run_threads(batch_fit, list(range(4)))
+        for i, (a, b) in enumerate(zip(par, serial)):
+            for x, y, name in zip(a, b, ("params", "crlb", "llr")):
+                np.testing.assert_array_equal(x, y, err_msg=f"round {rnd} batch {i} {name}")

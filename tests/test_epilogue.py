@@ -37,6 +37,11 @@ def test_chi2_pvalue_special_values(hiplib):
     assert hiplib.gmle_chi2_pvalue(float("inf"), 10) == 0.0
     assert np.isnan(hiplib.gmle_chi2_pvalue(float("nan"), 10))
     assert np.isnan(hiplib.gmle_chi2_pvalue(1.0, 0))
+    # chi^2 below zero (an LLR rounded just above zero): CDF 0, p = 1, as
+    # Distributions.jl's gamma CDF clamps x at 0 and scipy gives
+    for c in (-1e-3, -1.0, -1e30, float("-inf")):
+        assert hiplib.gmle_chi2_pvalue(c, 10) == 1.0
+        assert reference_pvalue(F32(c), 10) == 1.0
 
 
 def mirror(model, box, params, crlb, llr, xc, yc, psx, psy):
@@ -75,6 +80,8 @@ def test_epilogue_device_matches_reference_arithmetic(model, box):
     params = np.abs(rng.normal(5, 2, (n, P))).astype(F32)
     crlb = np.abs(rng.normal(0.05, 0.02, (n, P))).astype(F32)
     llr = (-rng.chisquare(box * box - P, n) / 2).astype(F32)
+    # LLRs rounded just above zero (chi^2 < 0) and exactly zero: p = 1
+    llr[:4] = np.array([5e-4, 0.5, 0.0, 1e-7], F32)
     xc = rng.integers(1, 2000, n).astype(np.int32)
     yc = rng.integers(1, 2000, n).astype(np.int32)
     d = lambda a: torch.from_numpy(a).cuda()  # noqa: E731
@@ -84,6 +91,7 @@ def test_epilogue_device_matches_reference_arithmetic(model, box):
     want = mirror(model, box, params, crlb, llr, xc, yc, 0.1, 0.107)
     for q in (0, 1, 2, 3, 5, 6, 7, 8, 9):
         np.testing.assert_array_equal(got[:, q], want[:, q], err_msg=f"field {q}")
+    assert np.all(got[:4, 4] == 1.0) and np.all(want[:4, 4] == 1.0)
     ulp = np.spacing(np.maximum(want[:, 4], F32(1e-30))).astype(np.float64)
     err = np.abs(got[:, 4].astype(np.float64) - want[:, 4]) / ulp
     print(f"[epilogue] model={model} box={box} p-value max {err.max():.1f} ulp, exact {np.mean(err == 0):.4f}",

@@ -491,3 +491,130 @@ def test_noiseless_fits_recover_truth_in_shifted_windows(oracle, model):
         # the data are the Float32-rounded expected counts: the optimum moves by
         # that rounding, ~1e-7 relative
         assert np.max(np.abs(got - want) / scale) < 2e-5, (dy, dx, np.max(np.abs(got - want) / scale))
+
+
+# ---------------------------------------------------------------------------
+# The Newton step itself (src/unified_kernel.jl:314-351, src/constraints.jl:49-54)
+# from the mathematics: for Poisson data the reference's gradient sum_p dudt cf
+# and diagonal sum_p (d2 cf - dudt^2 df) are the gradient and the diagonal of
+# the observed Hessian of the log-likelihood sum_p log Poisson(d_p; mu_p) (when
+# every mu > 0.01 and no 1e4 cap binds).  An independent step -- numpy init,
+# scipy's Poisson log-pmf, central finite differences, then
+# theta - clamp(g / h, -max_step, max_step) clamped to [lower, upper] -- must
+# reproduce the Float64 oracle after one and after two iterations.  The
+# constraints are chosen so that the step clamp binds for some parameters and
+# the box clamp for others; a swapped clamp order, a missing clamp or a sign
+# slip moves the result by far more than the finite-difference error.
+# ---------------------------------------------------------------------------
+def _np_init(roi, model):
+    n = roi.shape[0]
+    border = np.concatenate([roi[0, :], roi[-1, :], roi[1:-1, 0], roi[1:-1, -1]])
+    bg = border.mean()
+    s = np.maximum(0.0, roi - bg)
+    rows, cols = np.mgrid[1:n + 1, 1:n + 1]
+    tot = s.sum()
+    x = (s * cols).sum() / tot if tot > 10 else (n + 1) / 2
+    y = (s * rows).sum() / tot if tot > 10 else (n + 1) / 2
+    x, y = np.clip(x, 1.5, n - 0.5), np.clip(y, 1.5, n - 0.5)
+    th = [x, y, max(tot, 100.0), bg]
+    if model == XYNBSXSY:
+        th += [1.3, 1.3]
+    return np.array(th)
+
+
+def _fd_newton_step(th, roi, model, psf, lower, upper, max_step, var=None):
+    """One diagonal Newton step from finite differences of the log-likelihood.
+    Poisson: L = sum log Poisson(d; mu).  sCMOS (var given): the variance-
+    augmented Poisson L = sum (d + v) log(mu + v) - (mu + v), whose gradient is
+    the reference's sum dudt (d - mu) / (mu + v); its curvature term uses the
+    expected weight 1 / (mu + v) (camera_models.jl:36-42), i.e. the observed
+    diagonal plus sum dudt^2 (d - mu) / (mu + v)^2, with dudt from finite
+    differences of the model."""
+    box = roi.shape[0]
+
+    def loglik(t):
+        mu = _np_model(t, box, model, psf)
+        if var is None:
+            return np.sum(stats.poisson.logpmf(roi, mu))
+        return np.sum((roi + var) * np.log(mu + var) - (mu + var))
+
+    g = np.zeros(len(th))
+    h = np.zeros(len(th))
+    for k in range(len(th)):
+        # step sizes balancing truncation against rounding of a ~1e5 sum:
+        # x, y, bg 2e-4; N 0.05 (photons); widths 1e-4
+        e = {0: 2e-4, 1: 2e-4, 2: 0.05, 3: 2e-4}.get(k, 1e-4)
+        a, b = th.copy(), th.copy()
+        a[k] += e
+        b[k] -= e
+        la, l0, lb = loglik(a), loglik(th), loglik(b)
+        g[k] = (la - lb) / (2 * e)
+        h[k] = (la - 2 * l0 + lb) / (e * e)
+        if var is not None:
+            du = (_np_model(a, box, model, psf) - _np_model(b, box, model, psf)) / (2 * e)
+            mu = _np_model(th, box, model, psf)
+            h[k] += np.sum(du * du * (roi - mu) / (mu + var) ** 2)
+    step = np.where(np.abs(h) > 1e-10, g / h, 0.0)
+    return np.clip(th - np.clip(step, -max_step, max_step), lower, upper), g / h
+
+
+@pytest.mark.parametrize("model,camera", [(XYNB, "ideal"), (XYNBSXSY, "ideal"), (XYNB, "scmos")])
+def test_newton_step_matches_finite_difference_newton(oracle, model, camera):
+    rng = np.random.default_rng(70 + model + (5 if camera == "scmos" else 0))
+    box = 11
+    psf = [float(np.float32(1.3))] if model == XYNB else None
+    truth = [6.4, 5.7, 2500.0, 6.0] + ([1.45, 1.15] if model == XYNBSXSY else [])
+    mu = _np_model(np.array(truth), box, model, psf)
+    var = None
+    if camera == "scmos":
+        var = rng.lognormal(np.log(2.0), 0.5, (box, box))
+        roi = (rng.poisson(mu) + rng.normal(0, np.sqrt(var))).astype(np.float32).astype(np.float64)
+    else:
+        roi = rng.poisson(mu).astype(np.float32).astype(np.float64)
+    th0 = _np_init(roi, model)
+    # unconstrained first step, to place the constraints around it
+    P = len(th0)
+    _, raw = _fd_newton_step(th0, roi, model, psf, np.full(P, -np.inf), np.full(P, np.inf), np.full(P, np.inf), var)
+    lower = np.full(P, -1e30)
+    upper = np.full(P, 1e30)
+    max_step = np.full(P, 1e30)
+    # step clamp on x and N: half of the raw step
+    max_step[0] = abs(raw[0]) / 2
+    max_step[2] = abs(raw[2]) / 2
+    # box clamp on y: a bound halfway along the unclamped move
+    if raw[1] < 0:
+        upper[1] = th0[1] - raw[1] / 2
+    else:
+        lower[1] = th0[1] - raw[1] / 2
+    # bg starts outside its box (the initial estimate is not clamped,
+    # src/unified_kernel.jl:158-255), three clamped steps beyond the bound on
+    # the side it moves to: the step clamp, then the box clamp, lands on the
+    # bound; the other order would stop one clamped step from the start
+    max_step[3] = abs(raw[3]) / 4
+    if raw[3] > 0:
+        upper[3] = th0[3] - 3 * max_step[3]
+    else:
+        lower[3] = th0[3] + 3 * max_step[3]
+    lower, upper, max_step = (np.asarray(a, np.float32).astype(np.float64) for a in (lower, upper, max_step))
+    data = np.asfortranarray(roi.astype(np.float32)).reshape(-1, order="F")
+    vm = None if var is None else np.asfortranarray(var.astype(np.float32))
+    xc = yc = None if var is None else np.array([1], np.int32)
+    var32 = None if var is None else vm.astype(np.float64)
+    psf8 = psf if psf is not None else None
+    th_np = th0.copy()
+    for iters in (1, 2):
+        r = oracle.fit(model, psf8, box, data, lower.astype(np.float32), upper.astype(np.float32),
+                       max_step.astype(np.float32), iters, oracle.FP64, vm, xc, yc)
+        th_np, raw_k = _fd_newton_step(th_np, roi, model, psf, lower, upper, max_step, var32)
+        got = r["params"][0]
+        if iters == 1:
+            # both clamps bound on the first step
+            assert got[0] == pytest.approx(th0[0] - np.sign(raw[0]) * max_step[0], rel=1e-12, abs=1e-12)
+            assert got[2] == pytest.approx(th0[2] - np.sign(raw[2]) * max_step[2], rel=1e-12)
+            assert got[1] in (pytest.approx(lower[1]), pytest.approx(upper[1]))
+            assert got[3] == (upper[3] if raw[3] > 0 else lower[3])
+        # finite-difference error (measured: x, y ~1e-9 px, N ~1e-4, bg ~3e-6,
+        # widths ~1e-8); a clamp slip moves a parameter by a sizeable share of
+        # its ~0.02-85 step
+        tol = np.array([1e-7, 1e-7, 2e-3, 2e-5] + [1e-6] * (P - 4))
+        assert np.all(np.abs(got - th_np) <= tol), (iters, got - th_np)
