@@ -78,16 +78,18 @@ bool find_kernels(int model, int camera, int box, int lanes, int precision, Kern
 }
 
 namespace {
-thread_local std::string g_err;
+// The calling thread's last message, in a fixed buffer (recording an error
+// allocates nothing).
+thread_local char g_err[512];
+thread_local size_t g_err_len = 0;
 }  // namespace
 
 int fail(int code, const char* fmt, ...) {
-  char buf[512];
   va_list ap;
   va_start(ap, fmt);
-  vsnprintf(buf, sizeof(buf), fmt, ap);
+  const int n = vsnprintf(g_err, sizeof(g_err), fmt, ap);
   va_end(ap);
-  g_err = buf;
+  g_err_len = n < 0 ? 0 : std::min<size_t>((size_t)n, sizeof(g_err) - 1);
   return code;
 }
 
@@ -356,13 +358,15 @@ int enqueue_batch(const BatchSpec& b, hipStream_t stream) {
   p.llr = b.llr;
   p.crlb64 = b.precision == PREC_MIXED ? b.crlb64 : nullptr;
   p.llr64 = b.precision == PREC_MIXED ? b.llr64 : nullptr;
+  // Shape lists in fixed arrays: enqueueing a batch allocates nothing.
   int64_t slot = 0;
-  std::vector<int> shapes;
+  int shapes[kNumShapes];
+  int n_shapes = 0;
   for (int s = 0; s < kNumShapes; ++s) {
     p.start[s] = (int32_t)slot;
     p.count[s] = (int32_t)b.counts[s];
     slot += b.counts[s];
-    if (b.counts[s] > 0) shapes.push_back(s);
+    if (b.counts[s] > 0) shapes[n_shapes++] = s;
   }
   hipError_t e = enqueue_batch_plan(p, stream);
   if (e != hipSuccess) return hip_fail(e, "batch planning launch");
@@ -371,10 +375,16 @@ int enqueue_batch(const BatchSpec& b, hipStream_t stream) {
     // One launch for every shape the multi-shape kernel holds, heaviest tiles
     // first, with the requested lanes where it has them (else 4); the others
     // follow on the same stream with their own kernels.
-    std::vector<int> multi, rest;
-    for (int s : shapes)
-      (multi_eligible(s / kNumBoxes, b.camera, kMinBox + s % kNumBoxes) ? multi : rest).push_back(s);
-    std::sort(multi.begin(), multi.end(), [&](int u, int v) {
+    int multi[kNumShapes], rest[kNumShapes];
+    int n_multi = 0, n_rest = 0;
+    for (int q = 0; q < n_shapes; ++q) {
+      const int s = shapes[q];
+      if (multi_eligible(s / kNumBoxes, b.camera, kMinBox + s % kNumBoxes))
+        multi[n_multi++] = s;
+      else
+        rest[n_rest++] = s;
+    }
+    std::sort(multi, multi + n_multi, [&](int u, int v) {
       return shape_cost(u / kNumBoxes, kMinBox + u % kNumBoxes) > shape_cost(v / kNumBoxes, kMinBox + v % kNumBoxes);
     });
     MultiArgs m;
@@ -394,7 +404,8 @@ int enqueue_batch(const BatchSpec& b, hipStream_t stream) {
       a.offsets = b.offsets;
       a.out_stride = kBatchStride;
     }
-    for (int s : multi) {
+    for (int qm = 0; qm < n_multi; ++qm) {
+      const int s = multi[qm];
       const int model = s / kNumBoxes, box = kMinBox + s % kNumBoxes;
       const int want = (b.lanes && b.lanes[s] > 0) ? b.lanes[s] : large_batch_lanes(model, b.camera, box);
       const int q = m.n_entries++;
@@ -408,14 +419,15 @@ int enqueue_batch(const BatchSpec& b, hipStream_t stream) {
       e = b.camera == SCMOS ? launch_multi_scmos(m, stream) : launch_multi_ideal(m, stream);
       if (e != hipSuccess) return hip_fail(e, "kernel launch");
     }
-    shapes = rest;
+    for (int q = 0; q < n_rest; ++q) shapes[q] = rest[q];
+    n_shapes = n_rest;
   }
-  std::sort(shapes.begin(), shapes.end(), [&](int u, int v) {
+  std::sort(shapes, shapes + n_shapes, [&](int u, int v) {
     const double cu = b.counts[u] * shape_cost(u / kNumBoxes, kMinBox + u % kNumBoxes);
     const double cv = b.counts[v] * shape_cost(v / kNumBoxes, kMinBox + v % kNumBoxes);
     return cu > cv;
   });
-  const int ns = std::min<int>(b.streams, std::max<int>(1, (int)shapes.size()));
+  const int ns = std::min<int>(b.streams, std::max<int>(1, n_shapes));
   StreamSet* ss = nullptr;
   std::unique_lock<std::mutex> lock;
   hipStream_t strm[kMaxStreams] = {stream};
@@ -430,7 +442,8 @@ int enqueue_batch(const BatchSpec& b, hipStream_t stream) {
     }
   }
   double load[kMaxStreams] = {};
-  for (int s : shapes) {
+  for (int q = 0; q < n_shapes; ++q) {
+    const int s = shapes[q];
     const int model = s / kNumBoxes, box = kMinBox + s % kNumBoxes;
     int lanes = (b.lanes && b.lanes[s] > 0) ? b.lanes[s] : default_lanes(model, b.camera, box, b.precision, b.counts[s]);
     LaunchFn fn = find_launcher(model, b.camera, box, lanes, b.precision);
@@ -484,11 +497,11 @@ int gmle_device_count(void) {
 
 size_t gmle_last_error(char* buf, size_t len) {
   if (buf && len) {
-    const size_t n = g_err.size() < len - 1 ? g_err.size() : len - 1;
-    std::memcpy(buf, g_err.data(), n);
+    const size_t n = g_err_len < len - 1 ? g_err_len : len - 1;
+    std::memcpy(buf, g_err, n);
     buf[n] = '\0';
   }
-  return g_err.size();
+  return g_err_len;
 }
 
 int gmle_n_params(int model) {

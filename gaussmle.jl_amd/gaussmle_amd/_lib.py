@@ -84,6 +84,12 @@ SIGNATURES = {
          C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int, _I32, C.c_int, _I64, C.c_void_p, C.c_void_p,
          C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p],
     ),
+    "gmle_fit_batch_device_noalloc": (
+        C.c_int,
+        [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, _F, _F, _F, _F, C.c_int, C.c_void_p, C.c_int,
+         C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int, _I32, C.c_int, _I64, C.c_void_p, C.c_void_p,
+         C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
+    ),
 }
 
 

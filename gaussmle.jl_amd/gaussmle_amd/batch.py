@@ -230,6 +230,34 @@ def fit_batch_device(batch: MixedBatch, d: dict, precision: str = "fp32", lanes=
     return out
 
 
+def fit_batch_device_noalloc(batch: MixedBatch, d: dict, precision: str = "fp32", lanes=None, streams: int = 1,
+                             out: dict | None = None, stream=None, host_counts: bool = True) -> dict:
+    """gmle_fit_batch_device_noalloc: fit_batch_device with the library's
+    static device workspace instead of d["workspace"]; `out` must be given
+    (the call itself allocates nothing).  streams 0 or 1."""
+    import torch
+
+    lib = _lib.load()
+    s = stream if stream is not None else torch.cuda.current_stream(d["data"].device)
+    lo, up, ms = batch.constraint_tables()
+    psf = batch.psf_table()
+    ny, nx = (batch.var_map.shape if batch.camera else (0, 0))
+
+    def ptr(x):
+        return None if x is None else x.data_ptr()
+
+    counts = np.ascontiguousarray(d["counts"], dtype=np.int64) if host_counts else None
+    rc = lib.gmle_fit_batch_device_noalloc(
+        batch.n, ptr(d["models"]), ptr(d["boxes"]), ptr(d["offsets"]), ptr(d["data"]), _lib.fptr(psf.reshape(-1)),
+        _lib.fptr(lo.reshape(-1)), _lib.fptr(up.reshape(-1)), _lib.fptr(ms.reshape(-1)), batch.camera,
+        ptr(d.get("var")), ny, nx, ptr(d.get("xc")), ptr(d.get("yc")), batch.iterations,
+        _lib.PRECISION_IDS[precision], _lib.iptr(_lanes_table(lanes)), streams, _lib.i64ptr(counts),
+        ptr(out["params"]), ptr(out["crlb"]), ptr(out["llr"]), ptr(out.get("crlb64")), ptr(out.get("llr64")),
+        s.cuda_stream)
+    _lib.check(rc)
+    return out
+
+
 # ---------------------------------------------------------------------------
 # Synthetic mixed batches
 # ---------------------------------------------------------------------------

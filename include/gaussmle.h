@@ -182,6 +182,29 @@ int gmle_fit_batch_device(int64_t n_rois, const int8_t* d_models, const int8_t* 
                           float* d_params, float* d_crlb, float* d_llr, double* d_crlb64, double* d_llr64,
                           void* d_workspace, size_t workspace_bytes, void* stream);
 
+/* gmle_fit_batch_device with no allocation of any kind: no counterpart in the
+ * reference; for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The grouping
+ * workspace is a static device array of GMLE_NOALLOC_SCRATCH_BYTES (16 MiB)
+ * that every GPU the library runs on holds until the process ends; its address
+ * is looked up on the first call on each device.  Limits: n_rois up to the
+ * largest n with gmle_batch_workspace_bytes(n) <= GMLE_NOALLOC_SCRATCH_BYTES
+ * (4,194,176 ROIs); streams 0 (one launch) or 1 (shapes in order on `stream`);
+ * larger inputs or stream counts return GMLE_ERR_ARG.  Every call on a GPU
+ * shares that GPU's array, so calls that overlap on one GPU race: serialise
+ * them (one stream).  With shape_counts NULL the call waits for the device's
+ * shape count, as gmle_fit_batch_device; pass shape_counts for a call that
+ * never waits.  Results are bit-identical to gmle_fit_batch_device. */
+#define GMLE_NOALLOC_SCRATCH_BYTES (16u << 20)
+int gmle_fit_batch_device_noalloc(int64_t n_rois, const int8_t* d_models, const int8_t* d_boxes,
+                                  const int64_t* d_offsets, const float* d_data, const float* psf,
+                                  const float* lower, const float* upper, const float* max_step, int camera,
+                                  const float* d_var_map, int ny, int nx, const int32_t* d_x_corners,
+                                  const int32_t* d_y_corners, int iterations, int precision,
+                                  const int32_t* lanes, int streams, const int64_t* shape_counts,
+                                  float* d_params, float* d_crlb, float* d_llr, double* d_crlb64,
+                                  double* d_llr64, void* stream);
+
 /* ---------------------------------------------------------------------------
  * Result epilogue on the device: what GaussMLE.jl does per emitter on the host
  * after the fit (src/interface.jl:265-275, src/roi_batch.jl:29-34, 125-302).
