@@ -6,10 +6,16 @@ seeded synthetic ROIs.  The default workload is the headline one: 11x11
 GaussianXYNB (sigma 1.3 px), Poisson camera, 20 Newton iterations, with the
 parameter draws x, y = centre +- 1.5 px, N ~ U(500, 5000), bg ~ U(1, 20); the
 batch is 2^20 ROIs.  Prints one JSON line with fits/s, the kernel's roofline
-position (FP32 VALU, algorithmic FLOPs per fit as defined in DESIGN.md) and a
-CPU baseline (the oracle, timed on this host's cores on a bounded sample).
+position (FP32 VALU, algorithmic FLOPs per fit as flops_per_fit counts them)
+and a CPU baseline (the oracle, timed on this host's cores on a bounded
+sample).  Beside the headline the same line reports, each with its own
+roofline: the 13x13 sCMOS XYNBS and 15x15 astigmatic configurations at full
+size in both precisions, the uneven batch of 4,000,000 ROIs, the host-array
+entry point on the headline workload, the 10,000-ROI launch, the result
+epilogue, the streaming service, ROI extraction and the simulator.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload xynb11|xynbs13|astig15]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload xynb11|config1|config2|mixed|...]
+    python bench.py --trace-summary kernel_trace.csv     # per-kernel durations of a rocprofv3 trace
 """
 from __future__ import annotations
 
@@ -87,16 +93,18 @@ def mixed_bytes(mb) -> float:
 def run_mixed(n: int, steps: int, warmup: int, precision: str, tune_reps: int = 3, seed: int = 3):
     """Config-3 style uneven batch: autotune, then time the batched call
     against per-shape plain launches (each shape's ROIs contiguous, its best
-    lanes per ROI, launched one after another on one stream)."""
+    lanes per ROI, launched one after another on one stream).  The batch is
+    drawn on the device (synth_device.config3_device)."""
     import torch
     from gaussmle_amd import autotune
     from gaussmle_amd import batch as gb
+    from gaussmle_amd import synth_device
 
     t0 = time.perf_counter()
-    mb = gb.config3(n=n, seed=seed)
-    print(f"[bench] generated mixed batch of {n} ROIs in {time.perf_counter() - t0:.1f}s", file=sys.stderr,
-          flush=True)
-    d = gb.upload_batch(mb)
+    mb, d = synth_device.config3_device(n=n, seed=seed)
+    torch.cuda.synchronize()
+    print(f"[bench] generated mixed batch of {n} ROIs on the device in {time.perf_counter() - t0:.1f}s",
+          file=sys.stderr, flush=True)
     stacks = autotune.shape_stacks(mb, d)
     t0 = time.perf_counter()
     tuning = autotune.tune(mb, d, precision, reps=tune_reps, stacks=stacks)
@@ -148,7 +156,153 @@ def run_mixed(n: int, steps: int, warmup: int, precision: str, tune_reps: int = 
                          for s, c in enumerate(mb.shape_counts()) if c},
         "flops": mixed_flops(mb), "bytes": mixed_bytes(mb),
     }
-    return mb, res, dev_b
+    achieved = res["flops"] / dev_b / 1e12
+    res["roofline"] = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                       "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                       "kernel_ms": round(dev_b * 1e3, 4), "flops_per_fit": round(res["flops"] / n, 1),
+                       "note": "whole batched call (grouping kernels + every shape's fit kernel), HIP events"}
+    return mb, d, res, dev_b
+
+
+# Configurations reported beside the headline, drawn on the device.
+CONFIGS = {
+    "config1": dict(label="GaussianXYNBS 13x13, sCMOS 256x256 maps, 20 iterations", model="xynbs", box=13,
+                    iters=20, camera="scmos", rois=1_000_000, gen="config1_device"),
+    "config2": dict(label="AstigmaticXYZNB 15x15, Poisson, 30 iterations", model="astig", box=15, iters=30,
+                    camera="ideal", rois=4_000_000, gen="config2_device"),
+}
+
+
+def time_launches(fn, steps: int, warmup: int):
+    """Wall time of `steps` calls (synchronised on both sides) and the mean
+    HIP-event time per call on torch's current stream."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    t = time.perf_counter()
+    for a, b in ev:
+        a.record(s)
+        fn()
+        b.record(s)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t
+    return wall, float(np.mean([a.elapsed_time(b) for a, b in ev])) / 1e3
+
+
+def run_config(key: str, steps: int, warmup: int, n: int | None = None) -> dict:
+    """One of CONFIGS at its full size, in Float32 (one fit kernel) and in
+    mixed precision (fit kernel + Float64 final pass), with the roofline
+    position of each (algorithmic FLOPs as for the headline)."""
+    import torch
+    from gaussmle_amd import _lib
+    from gaussmle_amd import device as gd
+    from gaussmle_amd import synth_device
+    spec = CONFIGS[key]
+    t0 = time.perf_counter()
+    dw = getattr(synth_device, spec["gen"])(n=n or spec["rois"])
+    torch.cuda.synchronize()
+    print(f"[bench] {key}: generated {dw.n} ROIs ({spec['label']}) on the device in {time.perf_counter() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    lib = _lib.load()
+    ny, nx = (0, 0) if dw.var_map is None else dw.var_map.shape
+    F = flops_per_fit(spec["model"], dw.box, dw.iterations, spec["camera"])
+    res = {"workload": spec["label"], "rois": dw.n, "data": "synthetic: seeded parameters (numpy PCG64), pixels "
+           "drawn by the on-device simulator (Philox4x32-10), device-resident before timing"}
+    for precision in ("fp32", "mixed"):
+        lanes = lib.gmle_default_lanes(dw.psf_pixels.model_id, 0 if dw.camera == "ideal" else 1, dw.box,
+                                       _lib.PRECISION_IDS[precision], dw.n)
+        out = {}
+
+        def step():
+            out.update(gd.fit_device(dw.psf_pixels, dw.box, dw.d["data"], dw.constraints, dw.iterations, precision,
+                                     lanes, dw.d.get("var"), ny, nx, dw.d.get("xc"), dw.d.get("yc"),
+                                     out=out or None))
+
+        wall, dev_s = time_launches(step, steps, warmup)
+        achieved = F * dw.n / dev_s / 1e12
+        res[precision] = {
+            "fits_per_s": round(steps * dw.n / wall, 1), "ms_per_launch": round(dev_s * 1e3, 4), "lanes": lanes,
+            "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+                         "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                         "kernel_ms": round(dev_s * 1e3, 4), "flops_per_fit": F,
+                         "kernels": "fit_kernel" if precision == "fp32" else "fit_kernel + final_pass_kernel"}}
+        print(f"[bench] {key} {precision}: {res[precision]}", file=sys.stderr, flush=True)
+    del dw
+    torch.cuda.empty_cache()
+    return res
+
+
+def pcie_h2d_gbs(nbytes: int = 256 << 20, reps: int = 5) -> float:
+    """Host-to-device rate of one large copy from pinned memory: the ceiling
+    of any path that moves host arrays to the GPU."""
+    import torch
+    src = torch.empty(nbytes, dtype=torch.uint8).pin_memory()
+    dst = torch.empty(nbytes, dtype=torch.uint8, device="cuda:0")
+    dst.copy_(src, non_blocking=True)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        dst.copy_(src, non_blocking=True)
+    torch.cuda.synchronize()
+    rate = reps * nbytes / (time.perf_counter() - t) / 1e9
+    del src, dst
+    return rate
+
+
+def run_host_path(w, reps: int = 5) -> dict:
+    """The headline workload through gmle_fit on host arrays (the entry point
+    GaussMLE.jl's ccall uses): pageable input through the pinned upload
+    pipeline, fit, results back into the caller's arrays, per call wall time."""
+    import gaussmle_amd as g
+    from gaussmle_amd import _lib
+    f = g.GaussMLEFitter.__new__(g.GaussMLEFitter)
+    f.psf_model, f.iterations, f.constraints = w.psf_pixels, w.iterations, w.constraints
+    f.precision, f.lanes, f.batch_size, f.device = "fp32", 0, 10_000, g.GPU()
+    data = np.asfortranarray(w.data)
+    g.fit_arrays(f, data, w.psf_pixels)      # warm-up: buffers, pinned ring, worker threads
+    ts = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        g.fit_arrays(f, data, w.psf_pixels)
+        ts.append(time.perf_counter() - t)
+    dt = float(np.median(ts))
+    in_bytes = 4 * w.box * w.box
+    out_bytes = 4 * (2 * len(w.psf_pixels) + 1)
+    h2d = pcie_h2d_gbs()
+    bound = h2d * 1e9 / in_bytes
+    return {"entry_point": "gmle_fit (host arrays, synchronous)", "rois": w.n, "ms_per_call": round(dt * 1e3, 3),
+            "fits_per_s": round(w.n / dt, 1), "bytes_in_per_roi": in_bytes, "bytes_out_per_roi": out_bytes,
+            "pcie_h2d_gbs": round(h2d, 2), "pcie_bound_fits_per_s": round(bound, 1),
+            "fraction_of_pcie_bound": round(w.n / dt / bound, 4),
+            "vs_reference_gpu_host_call": round(w.n / dt / REFERENCE_GPU_FITS_PER_S, 3),
+            "host_threads": int(os.environ.get("GMLE_HOST_THREADS", min(8, os.cpu_count() or 1))),
+            "note": "median of 5 calls; pageable numpy arrays in and out"}
+
+
+def trace_summary(path: str, drop_first: int = 1) -> int:
+    """Per-kernel duration statistics of a rocprofv3 kernel trace (CSV),
+    leaving out each kernel's first `drop_first` dispatches (cold caches,
+    code-object load)."""
+    import csv
+    rows = {}
+    with open(path, newline="") as f:
+        for r in csv.DictReader(f):
+            name = r.get("Kernel_Name") or r.get("KernelName") or ""
+            t = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+            rows.setdefault(name, []).append(t)
+    out = []
+    for name, ts in rows.items():
+        kept = ts[drop_first:] if len(ts) > drop_first else ts
+        a = np.asarray(kept, dtype=np.float64)
+        out.append({"kernel": name[:160], "dispatches": len(ts), "used": len(kept), "mean_ns": round(a.mean(), 1),
+                    "median_ns": round(float(np.median(a)), 1), "min_ns": int(a.min()), "max_ns": int(a.max())})
+    out.sort(key=lambda r: -r["mean_ns"] * r["used"])
+    for r in out:
+        print(json.dumps(r))
+    return 0
 
 
 BOXER = {"ny": 2048, "nx": 2048, "boxsize": 11, "overlap": 2.0, "sigma_small": 1.0, "sigma_large": 2.0,
@@ -458,10 +612,11 @@ def main(argv=None) -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="xynb11", choices=sorted(WORKLOADS) + ["mixed", "serve", "boxer"])
+    ap.add_argument("--workload", default="xynb11",
+                    choices=sorted(WORKLOADS) + sorted(CONFIGS) + ["mixed", "serve", "boxer"])
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-lanes", type=int, default=2)
-    ap.add_argument("--mixed-rois", type=int, default=1_000_000,
+    ap.add_argument("--mixed-rois", type=int, default=4_000_000,
                     help="ROIs of the uneven batch reported beside the headline line (0: skip)")
     ap.add_argument("--serve-frames", type=int, default=300,
                     help="frames of the streaming-service run reported beside the headline line (0: skip)")
@@ -476,9 +631,16 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes (HBM counter calibration)")
+    ap.add_argument("--configs", default="config1,config2",
+                    help="configurations reported beside the headline line (comma list of "
+                         f"{','.join(CONFIGS)}; empty: none)")
+    ap.add_argument("--trace-summary", default=None, metavar="CSV",
+                    help="only summarise a rocprofv3 kernel trace (per-kernel durations, first dispatch dropped)")
     args = ap.parse_args(argv)
     if args.copy_probe:
         return copy_probe(args.copy_probe)
+    if args.trace_summary:
+        return trace_summary(args.trace_summary)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the per-ROI fit shards trivially across GPUs (independent ROIs), "
@@ -491,6 +653,10 @@ def main(argv=None) -> int:
 
     if args.workload == "mixed":
         return main_mixed(args)
+    if args.workload in CONFIGS:
+        res = run_config(args.workload, args.steps, args.warmup, args.rois or None)
+        print(json.dumps({"metric": METRIC, args.workload: res}), flush=True)
+        return 0
     if args.workload == "boxer":
         print(json.dumps({"metric": METRIC, "roi_extraction": run_boxer(args.boxer_frames or 32, args.steps,
                                                                        cpu=not args.no_cpu_baseline)}), flush=True)
@@ -589,9 +755,14 @@ def main(argv=None) -> int:
                "end_to_end": "fit loop + host p-value epilogue (chi-square CDF, scipy), median of 5",
                **host}
 
+    host_path = run_host_path(w) if args.workload == "xynb11" else None
+    configs = {}
+    if args.workload == "xynb11":
+        for key in [k for k in args.configs.split(",") if k]:
+            configs[key] = run_config(key, max(3, args.steps // 4), 2)
     batched = None
     if args.workload == "xynb11" and args.mixed_rois > 0:
-        _, batched, _ = run_mixed(args.mixed_rois, max(3, args.steps // 4), 2, args.precision)
+        _, _, batched, _ = run_mixed(args.mixed_rois, max(3, args.steps // 4), 2, args.precision)
     serving = None
     if args.workload == "xynb11" and args.serve_frames > 0:
         serving = run_service(args.serve_frames)
@@ -625,6 +796,8 @@ def main(argv=None) -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "config0_batch": small or None,
+        "host_path": host_path,
+        **configs,
         "epilogue": epilogue,
         "batched": batched,
         "serving": serving,
@@ -638,7 +811,7 @@ def main(argv=None) -> int:
 def main_mixed(args) -> int:
     """--workload mixed: the uneven batch (4M ROIs by default) is the line."""
     n = args.rois or 4_000_000
-    mb, res, dev_s = run_mixed(n, args.steps, args.warmup, args.precision)
+    mb, d, res, dev_s = run_mixed(n, args.steps, args.warmup, args.precision)
     achieved = res["flops"] / dev_s / 1e12
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
@@ -647,7 +820,9 @@ def main_mixed(args) -> int:
                 "note": "whole batched call (grouping kernels + every shape's fit kernel), HIP events"}
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline_mixed(mb, max(20_000, args.cpu_sample // 2), min(os.cpu_count() or 1, 16))
+        from gaussmle_amd import synth_device
+        k = max(20_000, args.cpu_sample // 2)
+        cpu = cpu_baseline_mixed(synth_device.host_batch(mb, d, k), k, min(os.cpu_count() or 1, 16))
     rec = {
         "metric": METRIC, "value": res["batched_fits_per_s"], "unit": "fits/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(n / res["batched_fits_per_s"] * 1e3, 4),

@@ -51,11 +51,24 @@ def _headers_mtime() -> float:
     return max(p.stat().st_mtime for p in hs)
 
 
+def _deps_mtime(obj: Path, fallback: float) -> float:
+    """Newest mtime among the files the object's depfile lists (hipcc -MMD),
+    or `fallback` (every header) when there is no usable depfile."""
+    dep = obj.with_suffix(".d")
+    try:
+        text = dep.read_text().replace("\\\n", " ")
+        files = text.split(":", 1)[1].split()
+        return max(Path(f).stat().st_mtime for f in files)
+    except (OSError, IndexError, ValueError):
+        return fallback
+
+
 def _compile(src: Path, force: bool, hdr_mtime: float) -> Path:
     obj = BUILD / (src.stem + ".o")
-    if not force and obj.exists() and obj.stat().st_mtime >= max(src.stat().st_mtime, hdr_mtime):
+    if not force and obj.exists() and obj.stat().st_mtime >= max(src.stat().st_mtime, _deps_mtime(obj, hdr_mtime)):
         return obj
-    cmd = [hipcc(), *HIPFLAGS, "-I", str(INCLUDE), "-c", str(src), "-o", str(obj)]
+    cmd = [hipcc(), *HIPFLAGS, "-I", str(INCLUDE), "-MMD", "-MF", str(obj.with_suffix(".d")), "-c", str(src), "-o",
+           str(obj)]
     multi = src.name.startswith("inst_multi_")
     if multi:
         cmd.append("-Rpass-analysis=kernel-resource-usage")

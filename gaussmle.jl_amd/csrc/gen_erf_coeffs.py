@@ -1,13 +1,15 @@
 """Generate the polynomial coefficients of the kernel's erf (gmle_erf.hpp).
 
-erf(u), u >= 0, is evaluated as one piece over [0, U_MAX]:
-    erf(u) = 1 - exp(-u^2) * R(min(u, U_MAX) - C),   R of degree DEG ~ erfcx
-erfcx(u) = exp(u^2) erfc(u) is smooth on [0, U_MAX]; the kernel already has
-exp(-u^2) at every pixel edge (it is the Gaussian of the derivative tables), so
-erf costs one polynomial and no transcendental of its own.  Beyond U_MAX,
-exp(-u^2) R < 3e-8 and 1 - exp(-u^2) R rounds to 1.0f by itself.  Near u = 0
-the form is accurate in absolute terms (~1e-7) rather than relative ones,
-which is what the pixel integrals Phi = (erf(b) - erf(a)) / 2 consume.
+The kernels work in the scaled variable v = k u, k = sqrt(log2 e), so that the
+Gaussian of every pixel edge is exp(-u^2) = exp2(-v^2) (one multiply and one
+v_exp_f32).  erfc(|u|) = exp(-u^2) erfcx(|u|), and erfcx(v / k) is fitted as
+one polynomial R of degree DEG in min(|v|, V_MAX) - C over [0, V_MAX]:
+    erfc(|u|) = exp2(-v^2) R(min(|v|, V_MAX) - C)
+erfcx is smooth there; V_MAX = k U_MAX with U_MAX = 5.6, where
+erfc(5.6) = 1.5e-15 lies below every tail a fit can see (N times a pixel
+integral against a background).  The kernels form pixel integrals from the
+edges' signed complements, so R's relative accuracy carries over to the
+tails.
 
 Fitting: least squares on Chebyshev nodes with Lawson's iteratively reweighted
 refinement toward the minimax (equioscillating) relative error, in float64;
@@ -19,9 +21,11 @@ measured on a dense grid.
 import numpy as np
 from scipy.special import erf, erfcx
 
-U_MAX = 3.92        # erf(u) rounds to 1.0f beyond this
-DEG = 15
-C = U_MAX / 2
+K = float(np.sqrt(np.log2(np.e)))   # v = K u
+U_MAX = 5.6
+V_MAX = K * U_MAX
+DEG = 16
+C = V_MAX / 2
 
 
 def lawson_fit(f, x, deg, center, iters=200):
@@ -50,17 +54,22 @@ def horner32(coef32, t32):
 def main():
     m = 20000
     nodes = (np.cos(np.pi * (np.arange(m) + 0.5) / m) + 1) / 2
-    u = nodes * U_MAX
-    c = lawson_fit(erfcx, u, DEG, C).astype(np.float32)
-    grid = np.linspace(0, U_MAX, 400001).astype(np.float32)
-    r = horner32(c, (grid - np.float32(C)).astype(np.float32))
-    e_rel = np.abs(r.astype(np.float64) / erfcx(grid.astype(np.float64)) - 1).max()
-    g32 = np.exp(-(grid.astype(np.float64) ** 2)).astype(np.float32)
-    v = (np.float32(1) - (g32.astype(np.float64) * r.astype(np.float64)).astype(np.float32)).astype(np.float32)
-    e_abs = np.abs(v.astype(np.float64) - erf(grid.astype(np.float64))).max()
-    print(f"// erfcx: max relative error {e_rel:.3e}; erf: max absolute error {e_abs:.3e}")
-    print(f"constexpr float kErfUMax = {U_MAX}f;")
-    print(f"constexpr float kErfC = {float(np.float32(C))!r}f;")
+    v = nodes * V_MAX
+    f = lambda x: erfcx(x / K)  # noqa: E731
+    c = lawson_fit(f, v, DEG, C).astype(np.float32)
+    vmax32, c32 = np.float32(V_MAX), np.float32(C)
+    grid = np.linspace(0, V_MAX, 400001).astype(np.float32)
+    r = horner32(c, (np.minimum(grid, vmax32) - c32).astype(np.float32))
+    e_rel = np.abs(r.astype(np.float64) / f(grid.astype(np.float64)) - 1).max()
+    ug = grid.astype(np.float64) / K
+    g32 = np.exp2(-(grid.astype(np.float64) ** 2)).astype(np.float32)
+    ec = (g32.astype(np.float64) * r.astype(np.float64)).astype(np.float32)
+    v_erf = (np.float32(1) - ec).astype(np.float32)
+    e_abs = np.abs(v_erf.astype(np.float64) - erf(ug)).max()
+    print(f"// erfcx: max relative error {e_rel:.3e}; erf = 1 - erfc: max absolute error {e_abs:.3e}")
+    print(f"constexpr float kErfK = {float(np.float32(K))!r}f;")
+    print(f"constexpr float kErfVMax = {float(vmax32)!r}f;")
+    print(f"constexpr float kErfC = {float(c32)!r}f;")
     print("constexpr float kErfR[%d] = {%s};" % (len(c), ", ".join(f"{x!r}f" for x in map(float, c))))
 
 

@@ -21,6 +21,7 @@
 #include "gmle_args.hpp"
 #include "gmle_batch.hpp"
 #include "gmle_host.hpp"
+#include "gmle_hostpipe.hpp"
 #include "gmle_multi.hpp"
 
 namespace gmle {
@@ -102,6 +103,7 @@ bool lanes_ok(int box, int lanes) { return lanes_built(box, lanes); }
 // Lanes per ROI for a large batch: the fewest lanes whose kernel still runs
 // two or more waves per SIMD without spilling (the batch autotuner measures).
 int large_batch_lanes(int model, int camera, int box) {
+  if (box > kMaxSmallBox) return large_box_lanes(model, camera, box);
   return box <= max_box_within_256(model, camera, 1) ? 1 : box <= max_box_within_256(model, camera, 2) ? 2 : 4;
 }
 
@@ -115,7 +117,7 @@ int default_lanes(int model, int camera, int box, int precision, int64_t n_rois)
   (void)precision;
   const int64_t fill = 256 * 4 * 64;   // CUs x SIMDs x lanes: one wave per SIMD
   int lanes = large_batch_lanes(model, camera, box);
-  if (!lanes_ok(box, lanes)) lanes = 2;
+  if (!lanes_ok(box, lanes)) lanes = box > kMaxSmallBox ? 16 : 2;
   for (int more = 2; more <= 16; more *= 2)
     if (lanes < more && n_rois * more <= fill && lanes_ok(box, more)) lanes = more;
   return lanes;
@@ -178,6 +180,7 @@ struct Cache {
   void* buf[kCacheSlots] = {};
   size_t cap[kCacheSlots] = {};
   hipStream_t stream = nullptr;
+  HostPipe pipe;   // pinned upload ring of the host-array entry points
 };
 Cache g_caches[kMaxDevices];
 
@@ -201,13 +204,61 @@ int current_cache(Cache** out) {
   return GMLE_OK;
 }
 
-// The cache's stream, created on first use (caller holds c.mu).
+// The cache's stream and upload pipeline, created on first use (caller holds c.mu).
 int cache_stream(Cache& c) {
   if (!c.stream) {
     hipError_t e = hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking);
     if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
   }
+  const hipError_t e = c.pipe.init();
+  if (e != hipSuccess) return hip_fail(e, "host pipeline set-up (pinned staging)");
   return GMLE_OK;
+}
+
+// Copies device outputs back through the pinned output buffer: the pieces
+// (device source, host destination, bytes) are queued on `s`, waited for and
+// copied to the caller's arrays by the worker pool.
+struct OutPiece {
+  const void* dev;
+  void* host;
+  size_t bytes;
+};
+int download(Cache& C, hipStream_t s, const OutPiece* pieces, int n) {
+  size_t total = 0;
+  for (int k = 0; k < n; ++k) total += (pieces[k].bytes + 255) & ~(size_t)255;
+  hipError_t e = C.pipe.ensure_out(total);
+  if (e != hipSuccess) return hip_fail(e, "hipHostMalloc (output staging)");
+  size_t at = 0;
+  for (int k = 0; k < n && e == hipSuccess; ++k) {
+    e = hipMemcpyAsync((char*)C.pipe.out + at, pieces[k].dev, pieces[k].bytes, hipMemcpyDeviceToHost, s);
+    at += (pieces[k].bytes + 255) & ~(size_t)255;
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync (download)");
+  at = 0;
+  for (int k = 0; k < n; ++k) {
+    copy_parallel(pieces[k].host, (char*)C.pipe.out + at, pieces[k].bytes);
+    at += (pieces[k].bytes + 255) & ~(size_t)255;
+  }
+  return GMLE_OK;
+}
+
+// Chunk ends (in floats) of an upload of n_rois ROIs whose ROI k ends at
+// end(k) floats: each chunk holds whole ROIs and at most kStageBytes.
+template <class EndFn>
+std::vector<size_t> chunk_cuts(int64_t n_rois, EndFn end) {
+  std::vector<size_t> cuts;
+  const size_t cap = kStageBytes / sizeof(float);
+  size_t start = 0;
+  for (int64_t k = 0; k < n_rois; ++k) {
+    const size_t e = end(k);
+    if (e - start > cap && k > 0 && end(k - 1) > start) {
+      cuts.push_back(end(k - 1));
+      start = end(k - 1);
+    }
+  }
+  cuts.push_back(end(n_rois - 1));
+  return cuts;
 }
 
 // ---------------------------------------------------------------------------
@@ -588,10 +639,6 @@ int gmle_fit(int model, const float* psf, int box, int64_t n_rois, const float* 
                         &lanes);
   if (rc != GMLE_OK) return rc;
   if (!data || !params || !crlb || !llr) return fail(GMLE_ERR_ARG, "data / output pointers must not be NULL");
-  const size_t npix = (size_t)n_rois * (size_t)box * (size_t)box;
-  // validate_fit_input (src/interface.jl:26-29): NaN / Inf rejected.
-  for (size_t q = 0; q < npix; ++q)
-    if (!std::isfinite(data[q])) return fail(GMLE_ERR_DATA, "Input data contains NaN or Inf values");
   if (camera == SCMOS) {
     if (!var_map || !x_corners || !y_corners)
       return fail(GMLE_ERR_ARG, "sCMOS fits need the variance map and ROI corners");
@@ -602,7 +649,13 @@ int gmle_fit(int model, const float* psf, int box, int64_t n_rois, const float* 
                     (long long)(k + 1), cx, cy, ny, nx);
     }
   }
-  if (gmle_device_count() <= 0) return fail(GMLE_ERR_NODEVICE, "no HIP device available");
+  // With a device the upload pipeline checks every value as it copies it; without one the data are
+  // still validated first, as validate_fit_input (src/interface.jl:26-29) does before any fit.
+  if (gmle_device_count() <= 0) {
+    if (!all_finite(data, (size_t)n_rois * (size_t)box * (size_t)box))
+      return fail(GMLE_ERR_DATA, "Input data contains NaN or Inf values");
+    return fail(GMLE_ERR_NODEVICE, "no HIP device available");
+  }
   LaunchFn fn = find_launcher(model, camera, box, lanes, precision);
   if (!fn) return fail(GMLE_ERR_UNSUPPORTED, "no kernel for model %d box %d lanes %d", model, box, lanes);
 
@@ -613,6 +666,8 @@ int gmle_fit(int model, const float* psf, int box, int64_t n_rois, const float* 
   if ((rc = cache_stream(C)) != GMLE_OK) return rc;
   hipError_t e = hipSuccess;
   const int P = n_params(model);
+  const size_t box2 = (size_t)box * (size_t)box;
+  const size_t npix = (size_t)n_rois * box2;
   const size_t nvar = camera == SCMOS ? (size_t)ny * (size_t)nx : 0;
   const bool want64 = precision == PREC_MIXED && (crlb64 || llr64);
   if ((e = ensure(C, 0, npix * sizeof(float))) != hipSuccess) return hip_fail(e, "hipMalloc data");
@@ -630,39 +685,56 @@ int gmle_fit(int model, const float* psf, int box, int64_t n_rois, const float* 
   hipStream_t s = C.stream;
   float* d_data = (float*)C.buf[0];
   int32_t* d_corners = (int32_t*)C.buf[5];
-  e = hipMemcpyAsync(d_data, data, npix * sizeof(float), hipMemcpyHostToDevice, s);
-  if (e == hipSuccess && camera == SCMOS) {
+  // the variance map and corners go first, on the work stream
+  if (camera == SCMOS) {
     e = hipMemcpyAsync(C.buf[4], var_map, nvar * sizeof(float), hipMemcpyHostToDevice, s);
     if (e == hipSuccess)
       e = hipMemcpyAsync(d_corners, x_corners, (size_t)n_rois * sizeof(int32_t), hipMemcpyHostToDevice, s);
     if (e == hipSuccess)
       e = hipMemcpyAsync(d_corners + n_rois, y_corners, (size_t)n_rois * sizeof(int32_t), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync (upload)");
   }
-  if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync (upload)");
-  FitArgs a;
-  fill_args(a, model, psf, lower, upper, max_step, n_rois, ny, nx, iterations);
-  a.data = d_data;
-  a.var_map = camera == SCMOS ? (const float*)C.buf[4] : nullptr;
-  a.x_corners = camera == SCMOS ? d_corners : nullptr;
-  a.y_corners = camera == SCMOS ? d_corners + n_rois : nullptr;
-  a.params = (float*)C.buf[1];
-  a.crlb = (float*)C.buf[2];
-  a.llr = (float*)C.buf[3];
-  a.crlb64 = want64 ? (double*)C.buf[6] : nullptr;
-  a.llr64 = want64 ? (double*)C.buf[7] : nullptr;
-  e = fn(a, s);
-  if (e != hipSuccess) return hip_fail(e, "kernel launch");
-  e = hipMemcpyAsync(params, a.params, (size_t)n_rois * P * sizeof(float), hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess) e = hipMemcpyAsync(crlb, a.crlb, (size_t)n_rois * P * sizeof(float), hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess) e = hipMemcpyAsync(llr, a.llr, (size_t)n_rois * sizeof(float), hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess && want64 && crlb64)
-    e = hipMemcpyAsync(crlb64, a.crlb64, (size_t)n_rois * P * sizeof(double), hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess && want64 && llr64)
-    e = hipMemcpyAsync(llr64, a.llr64, (size_t)n_rois * sizeof(double), hipMemcpyDeviceToHost, s);
-  if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync (download)");
-  e = hipStreamSynchronize(s);
-  if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
-  return GMLE_OK;
+  FitArgs base;
+  fill_args(base, model, psf, lower, upper, max_step, n_rois, ny, nx, iterations);
+  base.var_map = camera == SCMOS ? (const float*)C.buf[4] : nullptr;
+  base.params = (float*)C.buf[1];
+  base.crlb = (float*)C.buf[2];
+  base.llr = (float*)C.buf[3];
+  base.crlb64 = want64 ? (double*)C.buf[6] : nullptr;
+  base.llr64 = want64 ? (double*)C.buf[7] : nullptr;
+  // pixels stream in through the pinned ring; each chunk is fitted as soon as
+  // it is on the device, with the lanes chosen for the whole call (a ROI's
+  // result does not depend on which launch holds it)
+  const std::vector<size_t> cuts = chunk_cuts(n_rois, [&](int64_t k) { return (size_t)(k + 1) * box2; });
+  auto launch = [&](int k) -> int {
+    const int64_t r0 = k == 0 ? 0 : (int64_t)(cuts[k - 1] / box2), r1 = (int64_t)(cuts[k] / box2);
+    FitArgs a = base;
+    a.n_rois = r1 - r0;
+    a.data = d_data + (size_t)r0 * box2;
+    if (camera == SCMOS) {
+      a.x_corners = d_corners + r0;
+      a.y_corners = d_corners + n_rois + r0;
+    }
+    a.params += r0 * P;
+    a.crlb += r0 * P;
+    a.llr += r0;
+    if (a.crlb64) a.crlb64 += r0 * P;
+    if (a.llr64) a.llr64 += r0;
+    const hipError_t le = fn(a, s);
+    return le == hipSuccess ? GMLE_OK : hip_fail(le, "kernel launch");
+  };
+  rc = pipe_upload(C.pipe, data, d_data, cuts.data(), (int)cuts.size(), s, launch);
+  if (rc != GMLE_OK) {
+    (void)hipStreamSynchronize(s);
+    return rc;
+  }
+  OutPiece pieces[5] = {{base.params, params, (size_t)n_rois * P * sizeof(float)},
+                        {base.crlb, crlb, (size_t)n_rois * P * sizeof(float)},
+                        {base.llr, llr, (size_t)n_rois * sizeof(float)}};
+  int np = 3;
+  if (want64 && crlb64) pieces[np++] = {base.crlb64, crlb64, (size_t)n_rois * P * sizeof(double)};
+  if (want64 && llr64) pieces[np++] = {base.llr64, llr64, (size_t)n_rois * sizeof(double)};
+  return download(C, s, pieces, np);
 }
 
 int gmle_shape_index(int model, int box) { return shape_index(model, box); }
@@ -731,8 +803,8 @@ int gmle_fit_batch(int64_t n_rois, const int8_t* models, const int8_t* boxes, co
     const int m = models[k], box = boxes[k];
     if (m < XYNB || m > ASTIG) return fail(GMLE_ERR_ARG, "ROI %lld: unknown PSF model %d", (long long)(k + 1), m);
     if (box < kMinBox) return fail(GMLE_ERR_ARG, "ROI size must be at least 3x3, got %dx%d", box, box);
-    if (box > kMaxBox)
-      return fail(GMLE_ERR_UNSUPPORTED, "ROI size %d exceeds the supported maximum %d", box, kMaxBox);
+    if (box > kMaxBatchBox)
+      return fail(GMLE_ERR_UNSUPPORTED, "ROI size %d exceeds the mixed-batch maximum %d", box, kMaxBatchBox);
     if (camera == SCMOS) {
       const int cx = x_corners[k], cy = y_corners[k];
       if (cx < 1 || cy < 1 || cx > nx - box + 1 || cy > ny - box + 1)
@@ -743,14 +815,31 @@ int gmle_fit_batch(int64_t n_rois, const int8_t* models, const int8_t* boxes, co
     npix += (int64_t)box * box;
     ++counts[shape_index(m, box)];
   }
-  for (int64_t q = 0; q < npix; ++q)
-    if (!std::isfinite(data[q])) return fail(GMLE_ERR_DATA, "Input data contains NaN or Inf values");
   BatchSpec b{n_rois, nullptr, nullptr, nullptr, nullptr, psf, lower, upper, max_step, camera, nullptr, ny, nx,
               nullptr, nullptr, iterations, precision, lanes, streams, counts, nullptr, nullptr, nullptr,
               nullptr, nullptr, nullptr};
   int rc = check_batch(b);
   if (rc != GMLE_OK) return rc;
-  if (gmle_device_count() <= 0) return fail(GMLE_ERR_NODEVICE, "no HIP device available");
+  if (gmle_device_count() <= 0) {
+    if (!all_finite(data, (size_t)npix)) return fail(GMLE_ERR_DATA, "Input data contains NaN or Inf values");
+    return fail(GMLE_ERR_NODEVICE, "no HIP device available");
+  }
+
+  // chunks of whole ROIs through the pinned ring; each chunk is its own
+  // batch (grouped and fitted on the device as soon as it has landed)
+  const std::vector<size_t> cuts =
+      chunk_cuts(n_rois, [&](int64_t k) { return (size_t)(offsets[(size_t)k] + (int64_t)boxes[k] * boxes[k]); });
+  std::vector<int64_t> first(cuts.size() + 1, 0);
+  {
+    size_t c = 0;
+    for (int64_t k = 0; k < n_rois; ++k) {
+      const size_t e = (size_t)(offsets[(size_t)k] + (int64_t)boxes[k] * boxes[k]);
+      if (e > cuts[c]) first[++c] = k;
+    }
+    first[cuts.size()] = n_rois;
+  }
+  int64_t max_chunk = 0;
+  for (size_t c = 0; c < cuts.size(); ++c) max_chunk = std::max(max_chunk, first[c + 1] - first[c]);
 
   Cache* cache = nullptr;
   if ((rc = current_cache(&cache)) != GMLE_OK) return rc;
@@ -776,11 +865,10 @@ int gmle_fit_batch(int64_t n_rois, const int8_t* models, const int8_t* boxes, co
   if ((e = ensure(C, 8, n)) != hipSuccess) return hip_fail(e, "hipMalloc models");
   if ((e = ensure(C, 9, n)) != hipSuccess) return hip_fail(e, "hipMalloc boxes");
   if ((e = ensure(C, 10, n * sizeof(int64_t))) != hipSuccess) return hip_fail(e, "hipMalloc offsets");
-  if ((e = ensure(C, 11, batch_workspace(n_rois))) != hipSuccess) return hip_fail(e, "hipMalloc workspace");
+  if ((e = ensure(C, 11, batch_workspace(max_chunk))) != hipSuccess) return hip_fail(e, "hipMalloc workspace");
   hipStream_t s = C.stream;
   int32_t* d_corners = (int32_t*)C.buf[5];
-  e = hipMemcpyAsync(C.buf[0], data, (size_t)npix * sizeof(float), hipMemcpyHostToDevice, s);
-  if (e == hipSuccess) e = hipMemcpyAsync(C.buf[8], models, n, hipMemcpyHostToDevice, s);
+  e = hipMemcpyAsync(C.buf[8], models, n, hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipMemcpyAsync(C.buf[9], boxes, n, hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipMemcpyAsync(C.buf[10], offsets.data(), n * sizeof(int64_t), hipMemcpyHostToDevice, s);
   if (e == hipSuccess && camera == SCMOS) {
@@ -802,18 +890,53 @@ int gmle_fit_batch(int64_t n_rois, const int8_t* models, const int8_t* boxes, co
   b.crlb64 = want64 ? (double*)C.buf[6] : nullptr;
   b.llr64 = want64 ? (double*)C.buf[7] : nullptr;
   b.workspace = C.buf[11];
-  rc = enqueue_batch(b, s);
-  if (rc != GMLE_OK) return rc;
-  e = hipMemcpyAsync(params, b.params, n * kBatchStride * sizeof(float), hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess) e = hipMemcpyAsync(crlb, b.crlb, n * kBatchStride * sizeof(float), hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess) e = hipMemcpyAsync(llr, b.llr, n * sizeof(float), hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess && want64 && crlb64)
-    e = hipMemcpyAsync(crlb64, b.crlb64, n * kBatchStride * sizeof(double), hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess && want64 && llr64) e = hipMemcpyAsync(llr64, b.llr64, n * sizeof(double), hipMemcpyDeviceToHost, s);
-  if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync (download)");
-  e = hipStreamSynchronize(s);
-  if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
-  return GMLE_OK;
+  // lanes per shape fixed for the whole call (automatic choices from the
+  // whole batch's shape counts), so that a ROI's result does not depend on
+  // how the call is cut into chunks
+  // (the one-launch mode's automatic choice does not depend on counts)
+  int32_t call_lanes[kNumShapes];
+  for (int q = 0; q < kNumShapes; ++q) {
+    const int model = q / kNumBoxes, box = kMinBox + q % kNumBoxes;
+    const bool one_launch = streams == 0 && multi_eligible(model, camera, box);
+    call_lanes[q] = (lanes && lanes[q] > 0)      ? lanes[q]
+                    : (counts[q] > 0 && !one_launch) ? default_lanes(model, camera, box, precision, counts[q])
+                                                     : 0;
+  }
+  b.lanes = call_lanes;
+  int64_t chunk_counts[kNumShapes];
+  auto launch = [&](int c) -> int {
+    const int64_t r0 = first[(size_t)c], r1 = first[(size_t)c + 1];
+    std::fill(chunk_counts, chunk_counts + kNumShapes, (int64_t)0);
+    for (int64_t k = r0; k < r1; ++k) ++chunk_counts[shape_index(models[k], boxes[k])];
+    BatchSpec q = b;
+    q.n = r1 - r0;
+    q.models = b.models + r0;
+    q.boxes = b.boxes + r0;
+    q.offsets = b.offsets + r0;   // offsets stay global: ROI pixels at data + offsets[k]
+    if (camera == SCMOS) {
+      q.xc = b.xc + r0;
+      q.yc = b.yc + r0;
+    }
+    q.params = b.params + r0 * kBatchStride;
+    q.crlb = b.crlb + r0 * kBatchStride;
+    q.llr = b.llr + r0;
+    if (q.crlb64) q.crlb64 = b.crlb64 + r0 * kBatchStride;
+    if (q.llr64) q.llr64 = b.llr64 + r0;
+    q.counts = chunk_counts;
+    return enqueue_batch(q, s);
+  };
+  rc = pipe_upload(C.pipe, data, (float*)C.buf[0], cuts.data(), (int)cuts.size(), s, launch);
+  if (rc != GMLE_OK) {
+    (void)hipStreamSynchronize(s);
+    return rc;
+  }
+  OutPiece pieces[5] = {{b.params, params, n * kBatchStride * sizeof(float)},
+                        {b.crlb, crlb, n * kBatchStride * sizeof(float)},
+                        {b.llr, llr, n * sizeof(float)}};
+  int np = 3;
+  if (want64 && crlb64) pieces[np++] = {b.crlb64, crlb64, n * kBatchStride * sizeof(double)};
+  if (want64 && llr64) pieces[np++] = {b.llr64, llr64, n * sizeof(double)};
+  return download(C, s, pieces, np);
 }
 
 }  // extern "C"

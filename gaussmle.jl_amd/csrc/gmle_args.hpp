@@ -12,19 +12,25 @@ enum Camera { IDEAL = 0, SCMOS = 1 };
 enum Precision { PREC_FP32 = 0, PREC_MIXED = 1 };
 
 constexpr int kMinBox = 3;
-constexpr int kMaxBox = 16;
+constexpr int kMaxBox = 32;        // plain fits (gmle_fit, gmle_fit_device)
+constexpr int kMaxSmallBox = 16;   // above: the large-box kernels (8 or 16 lanes per ROI, rows outer)
+constexpr int kMaxBatchBox = 16;   // mixed batches, the service, ROI extraction
 constexpr int kWave = 64;
 
 __host__ __device__ constexpr int n_params(int model) {
   return model == XYNB ? 4 : model == XYNBS ? 5 : model == XYNBSXSY ? 6 : 5;
 }
 
-// The fit kernels built: every box size kMinBox..kMaxBox with 2, 4, 8 or 16
-// lanes per ROI, and with 1 lane up to 12 px, where one lane can still keep a
-// whole ROI resident in VGPRs.
+// The fit kernels built: every box size kMinBox..kMaxSmallBox with 2, 4, 8 or
+// 16 lanes per ROI, and with 1 lane up to 12 px, where one lane can still keep
+// a whole ROI resident in VGPRs; above kMaxSmallBox (the reference accepts any
+// box, src/interface.jl:22; its SMITE heritage's largest window is 21 px,
+// src/constants.jl:5) 8 or 16 lanes up to 24 px and 16 lanes up to kMaxBox,
+// so that a lane keeps at most 3 columns.
 __host__ __device__ constexpr bool lanes_built(int box, int lanes) {
-  return (box >= kMinBox && box <= kMaxBox) &&
-         ((lanes == 1 && box <= 12) || lanes == 2 || lanes == 4 || lanes == 8 || lanes == 16);
+  if (box < kMinBox || box > kMaxBox) return false;
+  if (box > kMaxSmallBox) return lanes == 16 || (lanes == 8 && box <= 24);
+  return (lanes == 1 && box <= 12) || lanes == 2 || lanes == 4 || lanes == 8 || lanes == 16;
 }
 
 // Largest box size whose kernel with `lanes` lanes per ROI still fits 256
@@ -42,9 +48,17 @@ __host__ __device__ constexpr int max_box_within_256(int model, int camera, int 
   if (lanes == 2)
     return model == XYNB ? (sc ? 11 : 14) : model == XYNBS ? (sc ? 9 : 11) : model == XYNBSXSY ? (sc ? 9 : 11)
                                                                                                : (sc ? 8 : 10);
-  if (lanes == 4) return (model == ASTIG && sc) ? 13 : kMaxBox;
-  if (lanes == 8) return model == XYNB ? kMaxBox : 15;
-  return kMaxBox;
+  if (lanes == 4) return (model == ASTIG && sc) ? 13 : kMaxSmallBox;
+  if (lanes == 8) return model == XYNB ? kMaxSmallBox : 15;
+  return kMaxSmallBox;
+}
+
+// Lanes per ROI of a large box (above kMaxSmallBox) in a large batch: 8 where
+// that kernel fits 256 VGPRs (two waves per SIMD), else 16.  From the gfx950
+// build's register counts: 8 lanes fit up to 24 px (216-256 VGPRs) except
+// astigmatic sCMOS ROIs above 20 px; 16 lanes fit up to 32 px (161-245).
+__host__ __device__ constexpr int large_box_lanes(int model, int camera, int box) {
+  return (box <= 24 && !(model == ASTIG && camera == SCMOS && box > 20)) ? 8 : 16;
 }
 
 // Passed by value to every fitting kernel.  Arrays follow the reference's
@@ -89,12 +103,13 @@ LaunchFn find_launcher(int model, int camera, int box, int lanes, int precision)
 
 // The kernels behind a launcher, for callers that build HIP graphs: the fit
 // kernel (grid: ceil(n / rois_per_block) blocks of kWave threads) and, for
-// mixed precision, the Float64 final pass (ceil(n / kWave) blocks); both take
-// one FitArgs by value.
+// mixed precision, the Float64 final pass (ceil(n / final_rois_per_block)
+// blocks); both take one FitArgs by value.
 struct KernelInfo {
   const void* fit;
   const void* final_pass;   // null for PREC_FP32
   int rois_per_block;
+  int final_rois_per_block; // the final pass's grid: ceil(n / final_rois_per_block) blocks
 };
 typedef bool (*DescribeFn)(KernelInfo*);
 bool find_kernels(int model, int camera, int box, int lanes, int precision, KernelInfo* out);

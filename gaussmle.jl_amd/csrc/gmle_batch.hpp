@@ -19,12 +19,13 @@
 
 namespace gmle {
 
-constexpr int kNumBoxes = kMaxBox - kMinBox + 1;
+constexpr int kNumBoxes = kMaxBatchBox - kMinBox + 1;
 constexpr int kNumShapes = 4 * kNumBoxes;
 constexpr int kBatchStride = 6;   // params / crlb values per ROI slot
 
 __host__ __device__ constexpr int shape_index(int model, int box) {
-  return (model < XYNB || model > ASTIG || box < kMinBox || box > kMaxBox) ? -1 : model * kNumBoxes + (box - kMinBox);
+  return (model < XYNB || model > ASTIG || box < kMinBox || box > kMaxBatchBox) ? -1
+                                                                             : model * kNumBoxes + (box - kMinBox);
 }
 
 struct BatchPlan {

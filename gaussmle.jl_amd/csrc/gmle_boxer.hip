@@ -406,7 +406,7 @@ Layout layout(int ny, int nx, int nf) {
 int check_params(int ny, int nx, int nf, int box, float overlap, float sigma_small, float sigma_large, float minval) {
   if (nf <= 0 || ny <= 0 || nx <= 0) return fail(GMLE_ERR_ARG, "frame stack %d x %d x %d is empty", ny, nx, nf);
   if (nf > 65535) return fail(GMLE_ERR_ARG, "at most 65535 frames per call, got %d", nf);
-  if (box < kMinBox || box > kMaxBox) return fail(GMLE_ERR_ARG, "boxsize %d out of range [%d, %d]", box, kMinBox, kMaxBox);
+  if (box < kMinBox || box > kMaxBatchBox) return fail(GMLE_ERR_ARG, "boxsize %d out of range [%d, %d]", box, kMinBox, kMaxBatchBox);
   if (ny < box || nx < box) return fail(GMLE_ERR_ARG, "frames of %d x %d are smaller than the box %d", ny, nx, box);
   if (!(overlap >= 0.0f) || !(overlap < (float)box)) return fail(GMLE_ERR_ARG, "overlap must be in [0, boxsize)");
   if (!(sigma_small > 0.0f) || !(sigma_large > 0.0f)) return fail(GMLE_ERR_ARG, "sigmas must be > 0");

@@ -66,19 +66,28 @@ template <> struct MT<ASTIG> {
 // ---------------------------------------------------------------------------
 template <class T> struct DM;
 template <> struct DM<float> {
-  // erf(u) and exp(-u^2) for u = t * s; the erf reuses the Gaussian (gmle_erf.hpp)
-  static __device__ __forceinline__ void erf_gauss(float t, float s, float& ev, float& g) {
-    const float u = t * s;
-    g = fast_exp2(-(u * u) * kErfLog2e);
-    ev = erf_with_gauss(u, g);
+  // Axis scales carry sqrt(log2 e): t * s is v = k u of gmle_erf.hpp, so the
+  // Gaussian exp(-u^2) is exp2(-v^2).
+  static __device__ __forceinline__ float erf_scale() { return kErfK; }
+  // exp(-u^2), and the edge's signed erfc and half-sign for the pixel
+  // integrals (gmle_erf.hpp), which reuse the Gaussian
+  static __device__ __forceinline__ void erf_gauss(float t, float s, float& se, float& hs, float& g) {
+    const float v = t * s;
+    g = fast_exp2((-v) * v);
+    erf_edge(v, g, se, hs);
   }
   static __device__ __forceinline__ float inv_sqrt_2pi() { return kInvSqrt2Pi; }
 };
 template <> struct DM<double> {
-  static __device__ __forceinline__ void erf_gauss(double t, double s, double& ev, double& g) {
+  static __device__ __forceinline__ double erf_scale() { return 1.0; }
+  // Float64 keeps the reference's erf difference: its cancellation (~1e-16
+  // absolute) is negligible, and tails that erf rounds to exactly +-1 stay
+  // exactly 0 as in the reference.  se = -erf(u), hs = 0 in pixel_integral.
+  static __device__ __forceinline__ void erf_gauss(double t, double s, double& se, double& hs, double& g) {
     const double u = t * s;
     g = exp(-(u * u));
-    ev = erf(u);
+    se = -erf(u);
+    hs = 0.0;
   }
   static __device__ __forceinline__ double inv_sqrt_2pi() { return kInvSqrt2PiD; }
 };
@@ -88,6 +97,7 @@ template <> struct DM<double> {
 // (pixel k covers [k+1/2, k+3/2] in the reference's 1-based coordinates).  For
 // pixel p with lower edge lo and upper edge hi (t = edge - mu):
 //   phi = (erf(t_hi*s) - erf(t_lo*s)) / 2                  baselibrary.jl:8-14
+//         (formed from signed complements, pixel_integral in gmle_erf.hpp)
 //   D   = c (g_hi - g_lo),  c = -N/(sqrt(2pi) sigma)        baselibrary.jl:21-33
 //   D2  = c/sigma^2 (t_hi g_hi - t_lo g_lo)
 //   S   = c/sigma   (t_hi g_hi - t_lo g_lo)                 baselibrary.jl:40-52
@@ -103,23 +113,23 @@ template <class T, int M> struct AxisTab {
 template <class T, int M, bool SIG>
 __device__ __forceinline__ void axis_tables(T mu, T sigma, T nph, int first, AxisTab<T, M>& tb) {
   const T inv_s = T(1) / sigma;
-  const T s_erf = sqrt(T(0.5) / (sigma * sigma));
+  const T s_erf = sqrt(T(0.5) / (sigma * sigma)) * DM<T>::erf_scale();
   const T c = -nph * DM<T>::inv_sqrt_2pi() * inv_s;
   const T c2 = c * inv_s * inv_s;
   const T cs = c * inv_s;
   const T css = c2 * inv_s * inv_s;
   const T m2s = T(-2) * inv_s;
-  T erf_prev = T(0), g_prev = T(0), tg_prev = T(0), t3g_prev = T(0);
+  T se_prev = T(0), hs_prev = T(0), g_prev = T(0), tg_prev = T(0), t3g_prev = T(0);
 #pragma unroll
   for (int e = 0; e <= M; ++e) {
     const T t = (T(first + e) + T(0.5)) - mu;
-    T ev, g;
-    DM<T>::erf_gauss(t, s_erf, ev, g);
+    T se, hs, g;
+    DM<T>::erf_gauss(t, s_erf, se, hs, g);
     const T tg = t * g;
     const T t3g = SIG ? t * t * tg : T(0);
     if (e > 0) {
       const int p = e - 1;
-      tb.phi[p] = T(0.5) * (ev - erf_prev);
+      tb.phi[p] = pixel_integral<T>(se_prev, hs_prev, se, hs);
       tb.D[p] = c * (g - g_prev);
       const T w = tg - tg_prev;
       tb.D2[p] = c2 * w;
@@ -128,7 +138,8 @@ __device__ __forceinline__ void axis_tables(T mu, T sigma, T nph, int first, Axi
         tb.SS[p] = m2s * tb.S[p] + css * (t3g - t3g_prev);
       }
     }
-    erf_prev = ev;
+    se_prev = se;
+    hs_prev = hs;
     g_prev = g;
     tg_prev = tg;
     t3g_prev = t3g;
@@ -141,14 +152,14 @@ template <class T> struct AxisK {
   T mu, s_erf, c, c2, cs, css, m2s;
 };
 template <class T> struct Edge {
-  T ev, g, tg, t3g;
+  T se, hs, g, tg, t3g;   // signed erfc, half-sign (gmle_erf.hpp), Gaussian, t g, t^3 g
 };
 template <class T>
 __device__ __forceinline__ AxisK<T> axis_consts(T mu, T sigma, T nph) {
   AxisK<T> k;
   const T inv_s = T(1) / sigma;
   k.mu = mu;
-  k.s_erf = sqrt(T(0.5) / (sigma * sigma));
+  k.s_erf = sqrt(T(0.5) / (sigma * sigma)) * DM<T>::erf_scale();
   k.c = -nph * DM<T>::inv_sqrt_2pi() * inv_s;
   k.c2 = k.c * inv_s * inv_s;
   k.cs = k.c * inv_s;
@@ -160,7 +171,7 @@ template <class T, bool SIG>
 __device__ __forceinline__ Edge<T> edge_eval(const AxisK<T>& k, int e) {
   Edge<T> r;
   const T t = (T(e) + T(0.5)) - k.mu;
-  DM<T>::erf_gauss(t, k.s_erf, r.ev, r.g);
+  DM<T>::erf_gauss(t, k.s_erf, r.se, r.hs, r.g);
   r.tg = t * r.g;
   r.t3g = SIG ? t * t * r.tg : T(0);
   return r;
@@ -172,7 +183,7 @@ template <class T> struct Pix {
 template <class T, bool SIG>
 __device__ __forceinline__ Pix<T> pix_from_edges(const AxisK<T>& k, const Edge<T>& lo, const Edge<T>& hi) {
   Pix<T> p;
-  p.phi = T(0.5) * (hi.ev - lo.ev);
+  p.phi = pixel_integral<T>(lo.se, lo.hs, hi.se, hi.hs);
   p.D = k.c * (hi.g - lo.g);
   const T w = hi.tg - lo.tg;
   p.D2 = k.c2 * w;
@@ -216,7 +227,8 @@ __device__ __forceinline__ Pix<float> slot_row(const AxisK<float>& kr, int r, co
                                                const Edge<float>& e_next) {
   const bool first = r == 0;
   Edge<float> hi;
-  hi.ev = group_next<L>(first ? e_next.ev : e_s.ev);
+  hi.se = group_next<L>(first ? e_next.se : e_s.se);
+  hi.hs = group_next<L>(first ? e_next.hs : e_s.hs);
   hi.g = group_next<L>(first ? e_next.g : e_s.g);
   hi.tg = group_next<L>(first ? e_next.tg : e_s.tg);
   hi.t3g = SIG ? group_next<L>(first ? e_next.t3g : e_s.t3g) : 0.f;
@@ -672,6 +684,83 @@ template <class TF, int P> struct FinalOut {
   TF llr;
 };
 
+// One pixel of the final pass (unified_kernel.jl:359-397): its LLR term
+// (camera_models.jl:60-85) and, where the model is > 0, its Fisher
+// contribution du du^T / v (v = model, or model + variance for sCMOS) to the
+// upper triangle Fu.  du holds the pixel's P model derivatives.
+template <int CAM, int P, class TF>
+__device__ __forceinline__ void final_pixel(TF d, TF var, TF mu, const TF* du, bool cvalid, TF* Fu, TF& llr) {
+  TF v = mu;
+  TF lp;
+  TF rmu = TF(0);   // 1/mu, Float64 ideal camera only
+  if constexpr (CAM == IDEAL) {
+    // camera_models.jl:60-71; in Float64 as d log(mu/d) + d - mu (one log)
+    if constexpr (sizeof(TF) == 8) {
+      // d log(mu/d) = -d log(d (1/mu)); 1/mu is also the Fisher weight below
+      rmu = mu > TF(0) ? TF(1) / mu : TF(0);
+      lp = (mu > TF(0) && d > TF(0)) ? -d * log(d * rmu) + (d - mu) : (mu > TF(0) && d == TF(0)) ? -mu : TF(0);
+    } else {
+      lp = (mu > TF(0) && d > TF(0)) ? d * logf(mu) - mu - (d * logf(d) - d)
+                                    : (mu > TF(0) && d == TF(0)) ? -mu : TF(0);
+    }
+  } else {
+    // camera_models.jl:73-85
+    v = mu + var;
+    const TF res = d - mu;
+    lp = v > TF(0) ? TF(-0.5) * (res * res / v) : TF(0);
+  }
+  if (cvalid) llr += lp;
+  if (cvalid && mu > TF(0)) {
+    const TF wv = (CAM == IDEAL && sizeof(TF) == 8) ? rmu : TF(1) / v;
+    int q = 0;
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      const TF ak = du[k] * wv;
+#pragma unroll
+      for (int l = k; l < P; ++l) Fu[q++] += ak * du[l];
+    }
+  }
+}
+
+// The pixel's model derivatives from its column and row tables
+// (psf_derivatives.jl:6-187): x, y, N, bg, and the sigma / z term.
+template <int MODEL, class TF>
+__device__ __forceinline__ void final_du(TF px, TF py, TF Dx, TF Dy, TF ax, TF ay, TF* du) {
+  using Tr = MT<MODEL>;
+  du[Tr::IX] = Dx * py;
+  du[Tr::IY] = Dy * px;
+  du[Tr::IN] = px * py;
+  du[Tr::IBG] = TF(1);
+  if constexpr (Tr::EXTRA == 2) {
+    du[Tr::IE1] = ax * py;
+    du[Tr::IE2] = ay * px;
+  } else if constexpr (Tr::EXTRA != 0) {
+    du[Tr::IE1] = ax * py + ay * px;
+  }
+}
+
+// Sums across the group, the symmetric Fisher matrix and its inversion.
+template <int L, int P, class TF>
+__device__ __forceinline__ FinalOut<TF, P> final_reduce(TF* Fu, TF llr) {
+  constexpr int NF = P * (P + 1) / 2;
+  FinalOut<TF, P> out;
+  out.llr = group_sum<L>(llr);
+#pragma unroll
+  for (int q = 0; q < NF; ++q) Fu[q] = group_sum<L>(Fu[q]);
+  TF F[P * P];
+  int q = 0;
+#pragma unroll
+  for (int k = 0; k < P; ++k)
+#pragma unroll
+    for (int l = k; l < P; ++l) {
+      F[k + P * l] = Fu[q];
+      F[l + P * k] = Fu[q];
+      ++q;
+    }
+  invert_fisher<P, TF>(F, out.crlb);
+  return out;
+}
+
 template <int MODEL, int CAM, int N, int L, class TF, class Src>
 __device__ __forceinline__ FinalOut<TF, MT<MODEL>::P> final_pass(const float* th, const float* psf, int c0,
                                                                const Src& src) {
@@ -708,52 +797,12 @@ __device__ __forceinline__ FinalOut<TF, MT<MODEL>::P> final_pass(const float* th
 #pragma unroll
     for (int i = 0; i < N; ++i) {
       const TF py = fy.phi[i];
-      const TF d = (TF)dcol[i];
-      const TF mu = bgf + nphf * px * py;
+      TF ay = TF(0);
+      if constexpr (SIG) ay = (MODEL == ASTIG) ? wf.dsy * fy.S[i] : fy.S[i];
       TF du[P];
-      du[Tr::IX] = fx.D * py;
-      du[Tr::IY] = fy.D[i] * px;
-      du[Tr::IN] = px * py;
-      du[Tr::IBG] = TF(1);
-      if constexpr (SIG) {
-        const TF ay = (MODEL == ASTIG) ? wf.dsy * fy.S[i] : fy.S[i];
-        if constexpr (Tr::EXTRA == 2) {
-          du[Tr::IE1] = ax * py;
-          du[Tr::IE2] = ay * px;
-        } else {
-          du[Tr::IE1] = ax * py + ay * px;
-        }
-      }
-      TF v = mu;
-      TF lp;
-      TF rmu = TF(0);   // 1/mu, Float64 ideal camera only
-      if constexpr (CAM == IDEAL) {
-        // camera_models.jl:60-71; in Float64 as d log(mu/d) + d - mu (one log)
-        if constexpr (sizeof(TF) == 8) {
-          // d log(mu/d) = -d log(d (1/mu)); 1/mu is also the Fisher weight below
-          rmu = mu > TF(0) ? TF(1) / mu : TF(0);
-          lp = (mu > TF(0) && d > TF(0)) ? -d * log(d * rmu) + (d - mu) : (mu > TF(0) && d == TF(0)) ? -mu : TF(0);
-        } else {
-          lp = (mu > TF(0) && d > TF(0)) ? d * logf(mu) - mu - (d * logf(d) - d)
-                                        : (mu > TF(0) && d == TF(0)) ? -mu : TF(0);
-        }
-      } else {
-        // camera_models.jl:73-85
-        v = mu + (TF)vcol[i];
-        const TF res = d - mu;
-        lp = v > TF(0) ? TF(-0.5) * (res * res / v) : TF(0);
-      }
-      if (cvalid) llr += lp;
-      if (cvalid && mu > TF(0)) {
-        const TF wv = (CAM == IDEAL && sizeof(TF) == 8) ? rmu : TF(1) / v;
-        int q = 0;
-#pragma unroll
-        for (int k = 0; k < P; ++k) {
-          const TF ak = du[k] * wv;
-#pragma unroll
-          for (int l = k; l < P; ++l) Fu[q++] += ak * du[l];
-        }
-      }
+      final_du<MODEL, TF>(px, py, fx.D, fy.D[i], ax, ay, du);
+      const TF mu = bgf + nphf * px * py;
+      final_pixel<CAM, P, TF>((TF)dcol[i], CAM == SCMOS ? (TF)vcol[i] : TF(0), mu, du, cvalid, Fu, llr);
     }
   };
   if constexpr (Src::kInRegisters) {
@@ -766,24 +815,65 @@ __device__ __forceinline__ FinalOut<TF, MT<MODEL>::P> final_pass(const float* th
 #pragma unroll 1
     for (int cc = 0; cc < CB; ++cc) column(cc);
   }
-  FinalOut<TF, P> out;
-  out.llr = group_sum<L>(llr);
+  return final_reduce<L, P, TF>(Fu, llr);
+}
+
+// The same final pass with rows in the outer loop, for large boxes: each
+// row's tables are built from its two edges where the row is used, and a lane
+// holds only its CB columns' tables, instead of every row's (which for
+// N > 16 would not fit the registers).  Pixels are register-resident.
+template <int MODEL, int CAM, int N, int L, class TF, class Src>
+__device__ __forceinline__ FinalOut<TF, MT<MODEL>::P> final_pass_rows(const float* th, const float* psf, int c0,
+                                                                    const Src& src) {
+  static_assert(Src::kInRegisters, "rows outer needs register-resident pixels");
+  using Tr = MT<MODEL>;
+  constexpr int P = Tr::P;
+  constexpr int CB = (N + L - 1) / L;
+  constexpr bool SIG = Tr::EXTRA != 0;
+  TF thf[P];
 #pragma unroll
-  for (int q = 0; q < NF; ++q) Fu[q] = group_sum<L>(Fu[q]);
-  TF F[P * P];
+  for (int k = 0; k < P; ++k) thf[k] = (TF)th[k];
+  Widths<TF> wf;
+  model_widths<MODEL, TF>(thf, psf, wf);
+  const TF nphf = thf[Tr::IN], bgf = thf[Tr::IBG];
+  const AxisK<TF> kx = axis_consts<TF>(thf[Tr::IX], wf.sx, nphf);
+  const AxisK<TF> ky = axis_consts<TF>(thf[Tr::IY], wf.sy, nphf);
+  Pix<TF> cx[CB];
+  TF ax[CB];
   {
-    int q = 0;
+    Edge<TF> lo = edge_eval<TF, false>(kx, c0);
 #pragma unroll
-    for (int k = 0; k < P; ++k)
-#pragma unroll
-      for (int l = k; l < P; ++l) {
-        F[k + P * l] = Fu[q];
-        F[l + P * k] = Fu[q];
-        ++q;
-      }
+    for (int cc = 0; cc < CB; ++cc) {
+      const Edge<TF> hi = edge_eval<TF, false>(kx, c0 + cc + 1);
+      cx[cc] = pix_from_edges<TF, SIG>(kx, lo, hi);
+      lo = hi;
+      ax[cc] = SIG ? ((MODEL == ASTIG) ? wf.dsx * cx[cc].S : cx[cc].S) : TF(0);
+    }
   }
-  invert_fisher<P, TF>(F, out.crlb);
-  return out;
+  constexpr int NF = P * (P + 1) / 2;
+  TF Fu[NF];
+#pragma unroll
+  for (int q = 0; q < NF; ++q) Fu[q] = TF(0);
+  TF llr = TF(0);
+  Edge<TF> rlo = edge_eval<TF, false>(ky, 0);
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const Edge<TF> rhi = edge_eval<TF, false>(ky, i + 1);
+    const Pix<TF> ry = pix_from_edges<TF, SIG>(ky, rlo, rhi);
+    rlo = rhi;
+    TF ay = TF(0);
+    if constexpr (SIG) ay = (MODEL == ASTIG) ? wf.dsy * ry.S : ry.S;
+#pragma unroll
+    for (int cc = 0; cc < CB; ++cc) {
+      const bool cvalid = (L == 1) || (c0 + cc < N);
+      TF du[P];
+      final_du<MODEL, TF>(cx[cc].phi, ry.phi, cx[cc].D, ry.D, ax[cc], ay, du);
+      const TF mu = bgf + nphf * cx[cc].phi * ry.phi;
+      final_pixel<CAM, P, TF>((TF)src.d(cc, i), CAM == SCMOS ? (TF)src.var(cc, i) : TF(0), mu, du, cvalid, Fu,
+                              llr);
+    }
+  }
+  return final_reduce<L, P, TF>(Fu, llr);
 }
 
 // Out-of-range sCMOS corner: no fit, NaN outputs.
@@ -1037,7 +1127,12 @@ __device__ __forceinline__ void fit_one(const FitArgs& a, const RoiRef& ref, boo
   // occupancy.
   if constexpr (PREC == PREC_FP32) {
     const RegSrc<N, CAM == SCMOS> srcr{dv, vr};
-    FinalOut<float, P> fo = final_pass<MODEL, CAM, N, L, float>(th, a.psf, c0, srcr);
+    FinalOut<float, P> fo;
+    if constexpr (N > kMaxSmallBox) {
+      fo = final_pass_rows<MODEL, CAM, N, L, float>(th, a.psf, c0, srcr);
+    } else {
+      fo = final_pass<MODEL, CAM, N, L, float>(th, a.psf, c0, srcr);
+    }
     if (!inb) nan_outputs(th, fo);
     if (active && r == 0) store_outputs<P>(a, ref.out, ref.stride, th, fo);
   } else {
@@ -1135,11 +1230,77 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(2))) void
   if (a.llr64) a.llr64[slot] = fo.llr;
 }
 
+// The Float64 final pass of large boxes (N > kMaxSmallBox): L lanes per ROI
+// as in the iteration kernel, each holding its column block in VGPRs (read
+// once from HBM in 16-byte runs), rows in the outer loop.
+template <int MODEL, int CAM, int N, int L>
+__global__ __launch_bounds__(kWave) void final_pass_lanes_kernel(FitArgs a) {
+  constexpr int P = MT<MODEL>::P;
+  constexpr int RPW = kWave / L;
+  constexpr int CB = (N + L - 1) / L;
+  const int lane = threadIdx.x;
+  const int r = lane & (L - 1);
+  const int c0 = r * CB;
+  if (a.n_rois <= 0 || (a.order && __builtin_amdgcn_readfirstlane(a.order[a.first]) < 0)) return;
+  for (int64_t base = (int64_t)blockIdx.x * RPW; base < a.n_rois; base += (int64_t)gridDim.x * RPW) {
+    RoiRef ref;
+    const bool active = locate_roi<MODEL, CAM, N>(a, base + lane / L, 0, ref);
+    const int64_t slot = ref.out;
+    float th[P];
+#pragma unroll
+    for (int k = 0; k < P; ++k) th[k] = a.params[slot * ref.stride + k];
+    bool inb = true;
+    if constexpr (CAM == SCMOS)
+      inb = (ref.cx >= 1) && (ref.cy >= 1) && (ref.cx <= a.nx - N + 1) && (ref.cy <= a.ny - N + 1);
+    float dv[CB * N];
+    float vr[CAM == SCMOS ? CB * N : 1];
+#pragma unroll
+    for (int cc = 0; cc < CB; ++cc) {
+      const int col = c0 + cc;
+      if (col < N) {
+        load_run<N>(ref.data + col * N, dv + cc * N);
+      } else {
+#pragma unroll
+        for (int i = 0; i < N; ++i) dv[cc * N + i] = 0.f;
+      }
+      if constexpr (CAM == SCMOS) {
+        if (col < N && inb) {
+          load_run<N>(a.var_map + (int64_t)(ref.cy - 1) + (int64_t)a.ny * (int64_t)(ref.cx - 1 + col), vr + cc * N);
+        } else {
+#pragma unroll
+          for (int i = 0; i < N; ++i) vr[cc * N + i] = 0.f;
+        }
+      }
+    }
+    const RegSrc<N, CAM == SCMOS> src{dv, vr};
+    FinalOut<double, P> fo = final_pass_rows<MODEL, CAM, N, L, double>(th, a.psf, c0, src);
+    if (!inb) nan_outputs(th, fo);
+    if (active && r == 0) {
+      store_outputs<P>(a, slot, ref.stride, th, fo);
+      if (a.crlb64) {
+#pragma unroll
+        for (int k = 0; k < P; ++k) a.crlb64[slot * ref.stride + k] = fo.crlb[k];
+        for (int k = P; k < ref.stride; ++k) a.crlb64[slot * ref.stride + k] = __builtin_nan("");
+      }
+      if (a.llr64) a.llr64[slot] = fo.llr;
+    }
+  }
+}
+
 template <int MODEL, int CAM, int N, int L, int PREC>
 bool describe_fit(KernelInfo* k) {
   k->fit = reinterpret_cast<const void*>(&fit_kernel<MODEL, CAM, N, L, PREC>);
-  k->final_pass = PREC == PREC_MIXED ? reinterpret_cast<const void*>(&final_pass_kernel<MODEL, CAM, N>) : nullptr;
   k->rois_per_block = kWave / L;
+  k->final_pass = nullptr;
+  k->final_rois_per_block = kWave;
+  if constexpr (PREC == PREC_MIXED) {
+    if constexpr (N > kMaxSmallBox) {
+      k->final_pass = reinterpret_cast<const void*>(&final_pass_lanes_kernel<MODEL, CAM, N, L>);
+      k->final_rois_per_block = kWave / L;
+    } else {
+      k->final_pass = reinterpret_cast<const void*>(&final_pass_kernel<MODEL, CAM, N>);
+    }
+  }
   return true;
 }
 
@@ -1150,8 +1311,13 @@ hipError_t launch_fit(const FitArgs& a, hipStream_t stream) {
   if (blocks <= 0) return hipSuccess;
   hipLaunchKernelGGL((fit_kernel<MODEL, CAM, N, L, PREC>), dim3((unsigned)blocks), dim3(kWave), 0, stream, a);
   if constexpr (PREC == PREC_MIXED) {
-    const int64_t fblocks = (a.n_rois + kWave - 1) / kWave;
-    hipLaunchKernelGGL((final_pass_kernel<MODEL, CAM, N>), dim3((unsigned)fblocks), dim3(kWave), 0, stream, a);
+    if constexpr (N > kMaxSmallBox) {
+      hipLaunchKernelGGL((final_pass_lanes_kernel<MODEL, CAM, N, L>), dim3((unsigned)blocks), dim3(kWave), 0, stream,
+                         a);
+    } else {
+      const int64_t fblocks = (a.n_rois + kWave - 1) / kWave;
+      hipLaunchKernelGGL((final_pass_kernel<MODEL, CAM, N>), dim3((unsigned)fblocks), dim3(kWave), 0, stream, a);
+    }
   }
   return hipGetLastError();
 }

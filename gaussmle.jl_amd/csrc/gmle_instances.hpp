@@ -2,9 +2,11 @@
 // specialisations built for one (model, camera, precision) triple, and the
 // lookup that maps run-time arguments onto them.
 //
-// Every box size 3..16 is built.  Lanes per ROI: 1, 2, 4, 8 and 16 up to
-// 12 px; 2 to 16 above, where one lane cannot keep a whole ROI resident in
-// VGPRs.  8 and 16 lanes serve small batches: shorter waves, more of them.
+// Every box size 3..32 is built (lanes_built).  Lanes per ROI: 1, 2, 4, 8
+// and 16 up to 12 px; 2 to 16 up to 16 px, where one lane cannot keep a whole
+// ROI resident in VGPRs; 8 and 16 up to 24 px and 16 above (large boxes, rows
+// in the outer loop).  8 and 16 lanes also serve small batches: shorter
+// waves, more of them.
 #pragma once
 
 #include "gmle_fit_kernel.hpp"
@@ -27,9 +29,15 @@ Fn pick_lanes(int lanes) {
   if constexpr (lanes_built(N, 1)) {
     if (lanes == 1) return entry<MODEL, CAM, PREC, N, 1, Describe>();
   }
-  if (lanes == 2) return entry<MODEL, CAM, PREC, N, 2, Describe>();
-  if (lanes == 4) return entry<MODEL, CAM, PREC, N, 4, Describe>();
-  if (lanes == 8) return entry<MODEL, CAM, PREC, N, 8, Describe>();
+  if constexpr (lanes_built(N, 2)) {
+    if (lanes == 2) return entry<MODEL, CAM, PREC, N, 2, Describe>();
+  }
+  if constexpr (lanes_built(N, 4)) {
+    if (lanes == 4) return entry<MODEL, CAM, PREC, N, 4, Describe>();
+  }
+  if constexpr (lanes_built(N, 8)) {
+    if (lanes == 8) return entry<MODEL, CAM, PREC, N, 8, Describe>();
+  }
   if (lanes == 16) return entry<MODEL, CAM, PREC, N, 16, Describe>();
   return nullptr;
 }
@@ -51,6 +59,22 @@ Fn pick(int box, int lanes) {
     case 14: return pick_lanes<MODEL, CAM, PREC, 14, Describe, Fn>(lanes);
     case 15: return pick_lanes<MODEL, CAM, PREC, 15, Describe, Fn>(lanes);
     case 16: return pick_lanes<MODEL, CAM, PREC, 16, Describe, Fn>(lanes);
+    case 17: return pick_lanes<MODEL, CAM, PREC, 17, Describe, Fn>(lanes);
+    case 18: return pick_lanes<MODEL, CAM, PREC, 18, Describe, Fn>(lanes);
+    case 19: return pick_lanes<MODEL, CAM, PREC, 19, Describe, Fn>(lanes);
+    case 20: return pick_lanes<MODEL, CAM, PREC, 20, Describe, Fn>(lanes);
+    case 21: return pick_lanes<MODEL, CAM, PREC, 21, Describe, Fn>(lanes);
+    case 22: return pick_lanes<MODEL, CAM, PREC, 22, Describe, Fn>(lanes);
+    case 23: return pick_lanes<MODEL, CAM, PREC, 23, Describe, Fn>(lanes);
+    case 24: return pick_lanes<MODEL, CAM, PREC, 24, Describe, Fn>(lanes);
+    case 25: return pick_lanes<MODEL, CAM, PREC, 25, Describe, Fn>(lanes);
+    case 26: return pick_lanes<MODEL, CAM, PREC, 26, Describe, Fn>(lanes);
+    case 27: return pick_lanes<MODEL, CAM, PREC, 27, Describe, Fn>(lanes);
+    case 28: return pick_lanes<MODEL, CAM, PREC, 28, Describe, Fn>(lanes);
+    case 29: return pick_lanes<MODEL, CAM, PREC, 29, Describe, Fn>(lanes);
+    case 30: return pick_lanes<MODEL, CAM, PREC, 30, Describe, Fn>(lanes);
+    case 31: return pick_lanes<MODEL, CAM, PREC, 31, Describe, Fn>(lanes);
+    case 32: return pick_lanes<MODEL, CAM, PREC, 32, Describe, Fn>(lanes);
   }
   return nullptr;
 }

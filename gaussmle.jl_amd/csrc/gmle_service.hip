@@ -54,7 +54,7 @@ constexpr int64_t kTargetBatchRois = 1 << 16;
 constexpr int64_t kXferBlocks = 256;
 // LDS staging of one chunk's raw counts: 16-byte vectors, plus one for a
 // start that is not 16-byte aligned.
-constexpr int kRawVecs = (kChunk * kMaxBox * kMaxBox * 2) / 16 + 1;
+constexpr int kRawVecs = (kChunk * kMaxBatchBox * kMaxBatchBox * 2) / 16 + 1;
 // Completion polling: spin this long, then sleep between polls.
 constexpr double kSpinSeconds = 2e-3;
 // Pinned memory the GPU reads and writes directly: mapped, visible to every
@@ -117,7 +117,7 @@ struct Chunk {                // up to kChunk consecutive ROIs of one request
 // Slack after a slot's pixels: reads stay inside the arena even if a caller
 // rewrote a request's box sizes after submitting it (results are then
 // meaningless, but nothing is read or written out of bounds).
-constexpr int64_t kPixSlack = (int64_t)kChunk * kMaxBox * kMaxBox + kMaxBox * kMaxBox;
+constexpr int64_t kPixSlack = (int64_t)kChunk * kMaxBatchBox * kMaxBatchBox + kMaxBatchBox * kMaxBatchBox;
 
 struct IngestArgs {
   const ReqDesc* reqs;
@@ -162,7 +162,7 @@ __global__ __launch_bounds__(kIngestBlock) void service_ingest(IngestArgs a) {
   // loads (over PCIe when they sit in pinned host memory, where wide requests
   // matter) into LDS.  The aligned span never leaves the pages (host) or the
   // arena and its slack (device) that hold the counts.
-  const int npix = min(c.npix, kChunk * kMaxBox * kMaxBox);
+  const int npix = min(c.npix, kChunk * kMaxBatchBox * kMaxBatchBox);
   const uintptr_t beg = (uintptr_t)(r.adu + c.pix0);
   const uintptr_t b0 = beg & ~(uintptr_t)15;
   const int nvec = (int)((beg + 2 * (uintptr_t)npix - b0 + 15) >> 4);
@@ -171,7 +171,7 @@ __global__ __launch_bounds__(kIngestBlock) void service_ingest(IngestArgs a) {
   if (t < c.count) {
     const int k = c.first + t;
     const int64_t roi = r.roi_base + k;
-    const int b = min(max((int)r.boxes[k], kMinBox), kMaxBox);
+    const int b = min(max((int)r.boxes[k], kMinBox), kMaxBatchBox);
     const int x = r.xc[k], y = r.yc[k];
     sbox[t] = b;
     sxc[t] = x;
@@ -200,7 +200,7 @@ __global__ __launch_bounds__(kIngestBlock) void service_ingest(IngestArgs a) {
     float* dst = a.data + r.pix_base + c.pix0 + p0;
     const int64_t m0 = (int64_t)(syc[q] - 1) + (int64_t)a.ny * (int64_t)(sxc[q] - 1);
 #pragma unroll
-    for (int u = 0; u < (kMaxBox * kMaxBox + kWave - 1) / kWave; ++u) {
+    for (int u = 0; u < (kMaxBatchBox * kMaxBatchBox + kWave - 1) / kWave; ++u) {
       const int p = lane + u * kWave;
       if (p < b * b) {
         const int i = p % b, j = p / b;   // row, column inside the ROI
@@ -613,7 +613,8 @@ int launch_slot(gmle_service* s, Slot& sl) {
     if ((e = set_kernel_node(sl.exec, n.fit, n.k.fit, gf, kWave, &fa)) != hipSuccess)
       return hip_fail(e, "hipGraphExecKernelNodeSetParams");
     if (n.fin) {
-      const unsigned gp = (unsigned)std::max<int64_t>(1, (cnt + kWave - 1) / kWave);
+      const unsigned gp =
+          (unsigned)std::max<int64_t>(1, (cnt + n.k.final_rois_per_block - 1) / n.k.final_rois_per_block);
       if ((e = set_kernel_node(sl.exec, n.fin, n.k.final_pass, gp, kWave, &fa)) != hipSuccess)
         return hip_fail(e, "hipGraphExecKernelNodeSetParams");
     }

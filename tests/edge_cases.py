@@ -9,8 +9,11 @@ the oracle must take the same branch, not merely land close:
   zero            all-zero ROIs: background 0, every model value only the
                   spot (tails below the 0.01 threshold); Cholesky
   constant        flat ROIs (no signal): N falls to its lower bound; Cholesky
-  point_fp        one hot pixel of 256 counts, PSF sigma 0.05 px, background
-                  free down to 0: only that pixel has model > 0, its
+  point_fp        one hot pixel of 256 counts, PSF sigma 0.02 px (every pixel
+                  edge lies beyond |u| = 17, where exp(-u^2) underflows in
+                  Float32 and erf rounds to +-1 in both precisions, so the
+                  pixel integrals are exactly 1 and 0), background free down
+                  to 0: only that pixel has model > 0, its
                   derivative vector is exactly (0, 0, 1, 1) with weight 2^-8,
                   so the regularised Fisher matrix holds the exact singular
                   block [[a, a], [a, a]] (a = 2^-8) beside two ridge-only
@@ -67,7 +70,7 @@ def fisher_cases(n: int = 64, seed: int = 500) -> dict:
     ii = rng.integers(3, box - 3, n)
     jj = rng.integers(3, box - 3, n)
     point[ii, jj, np.arange(n)] = 256.0
-    tiny = GaussianXYNB(0.05)
+    tiny = GaussianXYNB(0.02)
     out["point_fp_xynb_11"] = synth.Workload("point", tiny, box, point, nan_truth(4), _cons(tiny, {3: (0.0, 1e30)}),
                                              20)
 

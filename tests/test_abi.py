@@ -39,7 +39,14 @@ def test_scalar_queries(hiplib):
     assert hiplib.gmle_version().decode().startswith("gaussmle-hip")
     assert [hiplib.gmle_n_params(m) for m in range(4)] == [4, 5, 6, 5]
     assert hiplib.gmle_n_params(9) < 0
-    assert hiplib.gmle_max_box() == 16
+    assert hiplib.gmle_max_box() == 32
+    # large boxes: 8 or 16 lanes up to 24 px, 16 lanes above
+    assert hiplib.gmle_lanes_supported(21, 8) == 1 and hiplib.gmle_lanes_supported(21, 16) == 1
+    assert hiplib.gmle_lanes_supported(21, 4) == 0 and hiplib.gmle_lanes_supported(25, 8) == 0
+    assert hiplib.gmle_lanes_supported(32, 16) == 1 and hiplib.gmle_lanes_supported(33, 16) == 0
+    assert hiplib.gmle_default_lanes(0, 0, 21, 0, 10_000_000) == 8
+    assert hiplib.gmle_default_lanes(3, 1, 21, 0, 10_000_000) == 16
+    assert hiplib.gmle_default_lanes(1, 1, 28, 0, 10_000_000) == 16
     assert hiplib.gmle_lanes_supported(11, 1) == 1
     assert hiplib.gmle_lanes_supported(15, 1) == 0
     assert hiplib.gmle_lanes_supported(15, 4) == 1
@@ -91,7 +98,7 @@ def test_fit_rejects_bad_input_before_any_device_work(hiplib):
     d = np.ones(11 * 11 * 4, np.float32)
     assert _call_fit(hiplib, d, 11, 0) == _lib.GMLE_ERR_ARG                 # empty
     assert _call_fit(hiplib, d, 2, 4) == _lib.GMLE_ERR_ARG                  # box < 3
-    assert _call_fit(hiplib, np.ones(17 * 17, np.float32), 17, 1) == _lib.GMLE_ERR_UNSUPPORTED
+    assert _call_fit(hiplib, np.ones(33 * 33, np.float32), 33, 1) == _lib.GMLE_ERR_UNSUPPORTED
     assert _call_fit(hiplib, d, 11, 4, model=7) == _lib.GMLE_ERR_ARG
     assert _call_fit(hiplib, d, 11, 4, iterations=-1) == _lib.GMLE_ERR_ARG
     assert _call_fit(hiplib, d, 11, 4, lanes=3) == _lib.GMLE_ERR_UNSUPPORTED
