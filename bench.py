@@ -90,7 +90,8 @@ def mixed_bytes(mb) -> float:
     return float(np.sum(4 * b * b + 8 + 8 * P + 8))
 
 
-def run_mixed(n: int, steps: int, warmup: int, precision: str, tune_reps: int = 3, seed: int = 3):
+def run_mixed(n: int, steps: int, warmup: int, precision: str, tune_reps: int = 3, seed: int = 3,
+              batched_only: bool = False):
     """Config-3 style uneven batch: autotune, then time the batched call
     against per-shape plain launches (each shape's ROIs contiguous, its best
     lanes per ROI, launched one after another on one stream).  The batch is
@@ -138,17 +139,20 @@ def run_mixed(n: int, steps: int, warmup: int, precision: str, tune_reps: int = 
         return wall, float(np.mean([a.elapsed_time(b) for a, b in ev])) / 1e3
 
     wall_b, dev_b = timed(lambda: batched(tuning.streams))
-    wall_1, dev_1 = timed(lambda: batched(1))
-    wall_0, dev_0 = timed(lambda: batched(0))
-    wall_p, dev_p = timed(plain)
+    rate = lambda wall: None if wall is None else round(steps * n / wall, 1)  # noqa: E731
+    wall_1 = wall_0 = wall_p = dev_p = None
+    if not batched_only:
+        wall_1, _ = timed(lambda: batched(1))
+        wall_0, _ = timed(lambda: batched(0))
+        wall_p, dev_p = timed(plain)
     res = {
         "workload": MIXED_LABEL, "rois": n, "precision": precision,
-        "batched_fits_per_s": round(steps * n / wall_b, 1),
-        "batched_one_stream_fits_per_s": round(steps * n / wall_1, 1),
-        "one_launch_fits_per_s": round(steps * n / wall_0, 1),
-        "per_shape_plain_fits_per_s": round(steps * n / wall_p, 1),
-        "batched_over_plain": round(wall_p / wall_b, 4),
-        "batched_ms": round(dev_b * 1e3, 4), "plain_ms": round(dev_p * 1e3, 4),
+        "batched_fits_per_s": rate(wall_b),
+        "batched_one_stream_fits_per_s": rate(wall_1),
+        "one_launch_fits_per_s": rate(wall_0),
+        "per_shape_plain_fits_per_s": rate(wall_p),
+        "batched_over_plain": None if wall_p is None else round(wall_p / wall_b, 4),
+        "batched_ms": round(dev_b * 1e3, 4), "plain_ms": None if dev_p is None else round(dev_p * 1e3, 4),
         "streams": tuning.streams,
         "lanes": {f"{k[0]}:{k[1]}": v for k, v in tuning.lanes.items()},
         "plain_lanes": {f"{k[0]}:{k[1]}": v for k, v in best_alone.items()},
@@ -282,23 +286,31 @@ def run_host_path(w, reps: int = 5) -> dict:
             "note": "median of 5 calls; pageable numpy arrays in and out"}
 
 
-def trace_summary(path: str, drop_first: int = 1) -> int:
-    """Per-kernel duration statistics of a rocprofv3 kernel trace (CSV),
-    leaving out each kernel's first `drop_first` dispatches (cold caches,
-    code-object load)."""
+def trace_summary(path: str, drop_first: int = 1, last: int = 0) -> int:
+    """Per-kernel duration statistics of a rocprofv3 kernel trace (CSV), one
+    line per (kernel, grid size), leaving out each line's first `drop_first`
+    dispatches (cold caches, code-object load), or keeping only its last
+    `last` dispatches when `last` > 0 (the timed steps at the end of a run)."""
     import csv
     rows = {}
     with open(path, newline="") as f:
         for r in csv.DictReader(f):
             name = r.get("Kernel_Name") or r.get("KernelName") or ""
+            grid = "x".join(r[k] for k in ("Grid_Size_X", "Grid_Size_Y", "Grid_Size_Z") if k in r) \
+                or r.get("Grid_Size", "")
             t = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
-            rows.setdefault(name, []).append(t)
+            rows.setdefault((name, grid), []).append((int(r["Start_Timestamp"]), t))
     out = []
-    for name, ts in rows.items():
-        kept = ts[drop_first:] if len(ts) > drop_first else ts
-        a = np.asarray(kept, dtype=np.float64)
-        out.append({"kernel": name[:160], "dispatches": len(ts), "used": len(kept), "mean_ns": round(a.mean(), 1),
-                    "median_ns": round(float(np.median(a)), 1), "min_ns": int(a.min()), "max_ns": int(a.max())})
+    for (name, grid), ts in rows.items():
+        ts.sort()
+        if last > 0:
+            kept = ts[-last:]
+        else:
+            kept = ts[drop_first:] if len(ts) > drop_first else ts
+        a = np.asarray([t for _, t in kept], dtype=np.float64)
+        out.append({"kernel": name[:160], "grid": grid, "dispatches": len(ts), "used": len(kept),
+                    "mean_ns": round(a.mean(), 1), "median_ns": round(float(np.median(a)), 1),
+                    "min_ns": int(a.min()), "max_ns": int(a.max())})
     out.sort(key=lambda r: -r["mean_ns"] * r["used"])
     for r in out:
         print(json.dumps(r))
@@ -636,11 +648,16 @@ def main(argv=None) -> int:
                          f"{','.join(CONFIGS)}; empty: none)")
     ap.add_argument("--trace-summary", default=None, metavar="CSV",
                     help="only summarise a rocprofv3 kernel trace (per-kernel durations, first dispatch dropped)")
+    ap.add_argument("--trace-last", type=int, default=0, metavar="K",
+                    help="with --trace-summary: keep only each kernel's last K dispatches")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="--workload mixed: time only the tuned batched call (no one-stream, one-launch or "
+                         "per-shape plain comparisons)")
     args = ap.parse_args(argv)
     if args.copy_probe:
         return copy_probe(args.copy_probe)
     if args.trace_summary:
-        return trace_summary(args.trace_summary)
+        return trace_summary(args.trace_summary, last=args.trace_last)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the per-ROI fit shards trivially across GPUs (independent ROIs), "
@@ -811,7 +828,7 @@ def main(argv=None) -> int:
 def main_mixed(args) -> int:
     """--workload mixed: the uneven batch (4M ROIs by default) is the line."""
     n = args.rois or 4_000_000
-    mb, d, res, dev_s = run_mixed(n, args.steps, args.warmup, args.precision)
+    mb, d, res, dev_s = run_mixed(n, args.steps, args.warmup, args.precision, batched_only=args.batched_only)
     achieved = res["flops"] / dev_s / 1e12
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
