@@ -509,6 +509,12 @@ def run_service(n_frames: int = 300, submissions: int = 3000, loads=(0.1, 0.5, 0
     for _ in range(3):
         gb.fit_batch(mb, "fp32", t, 4)
     res["plain_host_path_fits_per_s"] = round(3 * mb.n / (time.perf_counter() - t1), 1)
+    # the service's peak against the batch kernel on device-resident ROIs and
+    # against the PCIe ceiling of its raw counts
+    pcie_bound = res["pcie_h2d_gbs"] * 1e9 / res["raw_bytes_per_roi"]
+    res["pcie_bound_fits_per_s"] = round(pcie_bound, 1)
+    res["peak_over_plain_batch"] = round(res["peak_fits_per_s"] / res["plain_batch_fits_per_s"], 4)
+    res["peak_over_pcie_bound"] = round(res["peak_fits_per_s"] / pcie_bound, 4)
     return res
 
 

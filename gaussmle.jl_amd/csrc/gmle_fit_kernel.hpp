@@ -77,6 +77,10 @@ template <> struct DM<float> {
     erf_edge(v, g, se, hs);
   }
   static __device__ __forceinline__ float inv_sqrt_2pi() { return kInvSqrt2Pi; }
+  // a / b for the per-iteration width and axis constants: one v_rcp_f32
+  // (1 ulp) and a multiply instead of the ~10-instruction IEEE division.
+  // The Newton step keeps the IEEE division.
+  static __device__ __forceinline__ float div(float a, float b) { return a * fast_rcp(b); }
 };
 template <> struct DM<double> {
   static __device__ __forceinline__ double erf_scale() { return 1.0; }
@@ -90,6 +94,7 @@ template <> struct DM<double> {
     hs = 0.0;
   }
   static __device__ __forceinline__ double inv_sqrt_2pi() { return kInvSqrt2PiD; }
+  static __device__ __forceinline__ double div(double a, double b) { return a / b; }
 };
 
 // ---------------------------------------------------------------------------
@@ -112,8 +117,8 @@ template <class T, int M> struct AxisTab {
 
 template <class T, int M, bool SIG>
 __device__ __forceinline__ void axis_tables(T mu, T sigma, T nph, int first, AxisTab<T, M>& tb) {
-  const T inv_s = T(1) / sigma;
-  const T s_erf = sqrt(T(0.5) / (sigma * sigma)) * DM<T>::erf_scale();
+  const T inv_s = DM<T>::div(T(1), sigma);
+  const T s_erf = sqrt(DM<T>::div(T(0.5), sigma * sigma)) * DM<T>::erf_scale();
   const T c = -nph * DM<T>::inv_sqrt_2pi() * inv_s;
   const T c2 = c * inv_s * inv_s;
   const T cs = c * inv_s;
@@ -157,9 +162,9 @@ template <class T> struct Edge {
 template <class T>
 __device__ __forceinline__ AxisK<T> axis_consts(T mu, T sigma, T nph) {
   AxisK<T> k;
-  const T inv_s = T(1) / sigma;
+  const T inv_s = DM<T>::div(T(1), sigma);
   k.mu = mu;
-  k.s_erf = sqrt(T(0.5) / (sigma * sigma)) * DM<T>::erf_scale();
+  k.s_erf = sqrt(DM<T>::div(T(0.5), sigma * sigma)) * DM<T>::erf_scale();
   k.c = -nph * DM<T>::inv_sqrt_2pi() * inv_s;
   k.c2 = k.c * inv_s * inv_s;
   k.cs = k.c * inv_s;
@@ -222,6 +227,26 @@ template <int MODEL, int CAM, int N, int L> constexpr bool share_rows() {
 template <int N, int L> constexpr int row_slots() { return (N + L - 1) / L; }   // rows per lane
 template <int N, int L> constexpr int edge_slots() { return (N + L) / L; }     // edges 0..N per lane
 
+// ---------------------------------------------------------------------------
+// Tail split.  With N = L q + 1 the plain column blocks give each lane
+// ceil(N / L) = q + 1 columns, and the group computes L (q + 1) columns for N:
+// almost a whole padding column per lane (13 px on 4 lanes: 52 pixel slots
+// per lane for 169 pixels).  Split instead, each lane holds q full columns
+// [r q, r q + q) and, of the last column N - 1, the rows r, r + L, r + 2L, ...
+// -- exactly the rows whose tables the lane builds itself when rows are
+// shared, so those pixels need no broadcast: q N + ceil(N / L) slots (13 px on
+// 4 lanes: 43).  The last column's partial row sums are folded with that
+// column's tables on every lane and summed across the group with the rest
+// (the fold is linear in the sums).
+// ---------------------------------------------------------------------------
+template <int MODEL, int CAM, int N, int L> constexpr bool tail_split() {
+  return L >= 2 && N > L && N % L == 1 && N <= kMaxSmallBox && share_rows<MODEL, CAM, N, L>();
+}
+// Register-resident full columns per lane.
+template <int MODEL, int CAM, int N, int L> constexpr int block_cols() {
+  return tail_split<MODEL, CAM, N, L>() ? N / L : (N + L - 1) / L;
+}
+
 template <int L, bool SIG>
 __device__ __forceinline__ Pix<float> slot_row(const AxisK<float>& kr, int r, const Edge<float>& e_s,
                                                const Edge<float>& e_next) {
@@ -258,11 +283,12 @@ __device__ __forceinline__ Pix<float> bcast_pix(const Pix<float>& p, int o) {
   return q;
 }
 
-// All N rows' tables on every lane, as axis_tables() builds them.
+// All N rows' tables on every lane, as axis_tables() builds them; own[s]
+// receives this lane's own row r + L s (the tail split uses them).
 template <int N, int L, bool SIG>
-__device__ __forceinline__ void shared_axis_tables(const AxisK<float>& kr, int r, AxisTab<float, N>& tb) {
+__device__ __forceinline__ void shared_axis_tables(const AxisK<float>& kr, int r, AxisTab<float, N>& tb,
+                                                   Pix<float>* own) {
   constexpr int RS = row_slots<N, L>();
-  Pix<float> own[RS];
   Edge<float> e_s = edge_eval<float, SIG>(kr, r);
 #pragma unroll
   for (int s = 0; s < RS; ++s) {
@@ -295,23 +321,23 @@ template <class T>
 __device__ __forceinline__ void astig_widths(T z, const float* psf, Widths<T>& w) {
   const T sx0 = (T)psf[0], sy0 = (T)psf[1], Ax = (T)psf[2], Ay = (T)psf[3];
   const T Bx = (T)psf[4], By = (T)psf[5], gamma = (T)psf[6], dd = (T)psf[7];
-  const T zdx = (z - gamma) / dd;
-  const T zdy = (z + gamma) / dd;
+  const T zdx = DM<T>::div(z - gamma, dd);
+  const T zdy = DM<T>::div(z + gamma, dd);
   const T zdx2 = zdx * zdx, zdy2 = zdy * zdy;
   const T ax = T(1) + zdx2 + Ax * (zdx2 * zdx) + Bx * (zdx2 * zdx2);
   const T ay = T(1) + zdy2 + Ay * (zdy2 * zdy) + By * (zdy2 * zdy2);
   const T sqax = sqrt(ax), sqay = sqrt(ay);
   w.sx = sx0 * sqax;
   w.sy = sy0 * sqay;
-  const T dax = (T(2) * zdx + T(3) * Ax * zdx2 + T(4) * Bx * (zdx2 * zdx)) / dd;
-  const T day = (T(2) * zdy + T(3) * Ay * zdy2 + T(4) * By * (zdy2 * zdy)) / dd;
-  const T hx = sx0 / (T(2) * sqax), hy = sy0 / (T(2) * sqay);
+  const T dax = DM<T>::div(T(2) * zdx + T(3) * Ax * zdx2 + T(4) * Bx * (zdx2 * zdx), dd);
+  const T day = DM<T>::div(T(2) * zdy + T(3) * Ay * zdy2 + T(4) * By * (zdy2 * zdy), dd);
+  const T hx = DM<T>::div(sx0, T(2) * sqax), hy = DM<T>::div(sy0, T(2) * sqay);
   w.dsx = hx * dax;
   w.dsy = hy * day;
-  const T d2ax = (T(2) + T(6) * Ax * zdx + T(12) * Bx * zdx2) / (dd * dd);
-  const T d2ay = (T(2) + T(6) * Ay * zdy + T(12) * By * zdy2) / (dd * dd);
-  w.d2sx = -(sx0 / (T(4) * (ax * sqax))) * (dax * dax) + hx * d2ax;
-  w.d2sy = -(sy0 / (T(4) * (ay * sqay))) * (day * day) + hy * d2ay;
+  const T d2ax = DM<T>::div(T(2) + T(6) * Ax * zdx + T(12) * Bx * zdx2, dd * dd);
+  const T d2ay = DM<T>::div(T(2) + T(6) * Ay * zdy + T(12) * By * zdy2, dd * dd);
+  w.d2sx = -DM<T>::div(sx0, T(4) * (ax * sqax)) * (dax * dax) + hx * d2ax;
+  w.d2sy = -DM<T>::div(sy0, T(4) * (ay * sqay)) * (day * day) + hy * d2ay;
 }
 
 template <int MODEL, class T>
@@ -368,25 +394,111 @@ __device__ __forceinline__ void fold_column(const Pix<float>& cx, const Widths<f
   }
 }
 
+// One pixel of the tail column (tail split): row ry of this lane's slot,
+// column tables folded later.  ok is false for slots past the last row; their
+// terms are zero.  For the Poisson fast path rs[] collects the row sums the q
+// form subtracts (phi, D, D2, A, AA, rows).
+template <int MODEL, int CAM, bool FAST>
+__device__ __forceinline__ void tail_pixel(const Pix<float>& ry, const Widths<float>& w, float d, float var, bool ok,
+                                           float ncx, float bg, float* St, float* rs) {
+  using Tr = MT<MODEL>;
+  constexpr bool SIG = Tr::EXTRA != 0;
+  constexpr bool TWO = (Tr::EXTRA == 1 || Tr::EXTRA == 3);
+  float ay = 0.f, aay = 0.f;
+  if constexpr (SIG) {
+    if constexpr (MODEL == ASTIG) {
+      ay = w.dsy * ry.S;
+      aay = ry.SS * (w.dsy * w.dsy) + ry.S * w.d2sy;
+    } else {
+      ay = ry.S;
+      aay = ry.SS;
+    }
+  }
+  const float py = ry.phi;
+  const float mu = fmaf(ncx, py, bg);
+  float cf, df;
+  if constexpr (CAM == IDEAL) {
+    const float rc = fast_rcp(mu);
+    const float q = d * rc;
+    if constexpr (FAST) {
+      cf = q;
+      df = q * rc;
+    } else {
+      const bool pos = mu > 0.01f;   // camera_models.jl:23-32
+      cf = pos ? jmin(q - 1.f, 1.0e4f) : 0.f;
+      df = pos ? jmin(q * rc, 1.0e4f) : 0.f;
+    }
+  } else {
+    const float v = mu + var;   // camera_models.jl:36-42
+    const float rc = fast_rcp(v);
+    cf = (d - mu) * rc;
+    df = rc;
+  }
+  cf = ok ? cf : 0.f;
+  df = ok ? df : 0.f;
+  St[0] = fmaf(py, cf, St[0]);
+  St[1] = fmaf(py * py, df, St[1]);
+  St[2] = fmaf(ry.D, cf, St[2]);
+  St[3] = fmaf(ry.D2, cf, St[3]);
+  St[4] = fmaf(ry.D * ry.D, df, St[4]);
+  St[5] += cf;
+  St[6] += df;
+  if constexpr (SIG) {
+    St[7] = fmaf(ay, cf, St[7]);
+    St[8] = fmaf(aay, cf, St[8]);
+    St[10] = fmaf(ay * ay, df, St[10]);
+    if constexpr (TWO) St[9] = fmaf(py * ay, df, St[9]);
+  }
+  if constexpr (CAM == IDEAL && FAST) {
+    rs[0] += ok ? py : 0.f;
+    rs[1] += ok ? ry.D : 0.f;
+    rs[2] += ok ? ry.D2 : 0.f;
+    rs[3] += ok ? ay : 0.f;
+    rs[4] += ok ? aay : 0.f;
+    rs[5] += ok ? 1.f : 0.f;
+  }
+}
+
+// The tail column's sums folded into G / H with its tables tl.
+template <int MODEL, int CAM, bool FAST>
+__device__ __forceinline__ void tail_fold(float* St, const float* rs, const Pix<float>& tl, const Widths<float>& w,
+                                          float* G, float* H) {
+  if constexpr (CAM == IDEAL && FAST) {
+    St[0] -= rs[0];
+    St[2] -= rs[1];
+    St[3] -= rs[2];
+    St[5] -= rs[5];
+    if constexpr (MT<MODEL>::EXTRA != 0) {
+      St[7] -= rs[3];
+      St[8] -= rs[4];
+    }
+  }
+  fold_column<MODEL>(tl, w, St, G, H);
+}
+
 // The same sums with rows in the outer loop: each row's tables are built on
 // the fly from its two edges and the CB columns of this lane keep their S
 // vectors live instead.  Cheaper in VGPRs than holding every row's tables
 // when there are many row tables (sigma / z models) and few columns per lane.
 template <int MODEL, int CAM, int N, int L, bool FAST, bool SHARE>
-__device__ __forceinline__ void accumulate_rows(const float* dv, const float* vr, int c0, float nph, float bg,
-                                                const AxisK<float>& kr, int r, const AxisK<float>& kc,
-                                                const Widths<float>& w, float* G, float* H) {
+__device__ __forceinline__ void accumulate_rows(const float* dv, const float* vr, const float* dt, const float* vt,
+                                                int c0, float nph, float bg, const AxisK<float>& kr, int r,
+                                                const AxisK<float>& kc, const Widths<float>& w, float* G, float* H) {
   using Tr = MT<MODEL>;
   constexpr int P = Tr::P;
-  constexpr int CB = (N + L - 1) / L;
+  constexpr bool TAIL = tail_split<MODEL, CAM, N, L>();
+  constexpr int CB = block_cols<MODEL, CAM, N, L>();
   constexpr bool SIG = Tr::EXTRA != 0;
   constexpr bool TWO = (Tr::EXTRA == 1 || Tr::EXTRA == 3);
   constexpr int NS = SIG ? 11 : 7;
+  static_assert(!TAIL || SHARE, "the tail split uses the lane's own row tables");
 #pragma unroll
   for (int k = 0; k < P; ++k) { G[k] = 0.f; H[k] = 0.f; }
-  // this lane's column tables
+  // this lane's column tables (and the tail column's, from lane L - 1, whose
+  // block ends at edge N - 1)
   Pix<float> cx[CB];
   float ncx[CB];
+  Pix<float> tl{};
   {
     Edge<float> lo = edge_eval<float, SIG>(kc, c0);
 #pragma unroll
@@ -396,7 +508,17 @@ __device__ __forceinline__ void accumulate_rows(const float* dv, const float* vr
       lo = hi;
       ncx[cc] = nph * cx[cc].phi;
     }
+    if constexpr (TAIL) {
+      const Edge<float> hi = edge_eval<float, SIG>(kc, N);
+      tl = bcast_pix<L, SIG>(pix_from_edges<float, SIG>(kc, lo, hi), L - 1);
+    }
   }
+  const float ncx_t = nph * tl.phi;
+  float St[11], rs[6];
+#pragma unroll
+  for (int q = 0; q < 11; ++q) St[q] = 0.f;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) rs[q] = 0.f;
   float S[CB][11];
 #pragma unroll
   for (int cc = 0; cc < CB; ++cc)
@@ -423,6 +545,10 @@ __device__ __forceinline__ void accumulate_rows(const float* dv, const float* vr
         const Edge<float> e_next = next_slot_edge<N, L, SIG>(kr, r, i / L, rlo);
         own = slot_row<L, SIG>(kr, r, rlo, e_next);
         rlo = e_next;
+        // the tail pixel of this slot's row r + i, with the lane's own tables
+        if constexpr (TAIL)
+          tail_pixel<MODEL, CAM, FAST>(own, w, dt[i / L], CAM == SCMOS ? vt[i / L] : 0.f, r + i < N, ncx_t, bg, St,
+                                       rs);
       }
       ry = bcast_pix<L, SIG>(own, i % L);
     }
@@ -503,11 +629,12 @@ __device__ __forceinline__ void accumulate_rows(const float* dv, const float* vr
     // Columns past the ROI (last lane of a group when L does not divide N)
     // contribute nothing.  Selects, not a branch: a per-lane branch here
     // splits the block and costs ~100 VGPRs of live ranges.
-    const bool cvalid = (L == 1) || (c0 + cc < N);
+    const bool cvalid = (L == 1) || TAIL || (c0 + cc < N);
 #pragma unroll
     for (int q = 0; q < 11; ++q) S[cc][q] = cvalid ? S[cc][q] : 0.f;
     fold_column<MODEL>(cx[cc], w, S[cc], G, H);
   }
+  if constexpr (TAIL) tail_fold<MODEL, CAM, FAST>(St, rs, tl, w, G, H);
   (void)NS;
 }
 
@@ -548,7 +675,8 @@ __device__ __forceinline__ void accumulate(const float* dv, const float* vr, int
                                            float* H) {
   using Tr = MT<MODEL>;
   constexpr int P = Tr::P;
-  constexpr int CB = (N + L - 1) / L;
+  constexpr bool TAIL = tail_split<MODEL, CAM, N, L>();
+  constexpr int CB = block_cols<MODEL, CAM, N, L>();
   constexpr bool SIG = Tr::EXTRA != 0;
   constexpr bool TWO = (Tr::EXTRA == 1 || Tr::EXTRA == 3);
 #pragma unroll
@@ -559,7 +687,7 @@ __device__ __forceinline__ void accumulate(const float* dv, const float* vr, int
     const Edge<float> hi = edge_eval<float, SIG>(kc, c0 + cc + 1);
     const Pix<float> cx = pix_from_edges<float, SIG>(kc, lo, hi);
     lo = hi;
-    const bool cvalid = (L == 1) || (c0 + cc < N);
+    const bool cvalid = (L == 1) || TAIL || (c0 + cc < N);
     const float nphx = nph * cx.phi;
     float S1 = 0.f, S2 = 0.f, S3 = 0.f, S4 = 0.f, S5 = 0.f, S6 = 0.f, S7 = 0.f;
     float S8 = 0.f, S9 = 0.f, S10 = 0.f, S11 = 0.f;
@@ -626,6 +754,8 @@ template <int N, bool SC> struct RegSrc {
   static constexpr bool kInRegisters = true;
   const float* dv;   // this lane's column block, [cc * N + i]
   const float* vr;
+  const float* dt = nullptr;   // tail split: this lane's rows r + L s of the last column
+  const float* vt = nullptr;
   __device__ __forceinline__ float d(int cc, int i) const { return dv[cc * N + i]; }
   __device__ __forceinline__ float var(int cc, int i) const { return SC ? vr[cc * N + i] : 0.f; }
   __device__ __forceinline__ void column(int cc, float* dcol, float* vcol) const {
@@ -761,12 +891,12 @@ __device__ __forceinline__ FinalOut<TF, P> final_reduce(TF* Fu, TF llr) {
   return out;
 }
 
-template <int MODEL, int CAM, int N, int L, class TF, class Src>
+template <int MODEL, int CAM, int N, int L, class TF, class Src, bool TAIL = false>
 __device__ __forceinline__ FinalOut<TF, MT<MODEL>::P> final_pass(const float* th, const float* psf, int c0,
-                                                               const Src& src) {
+                                                               const Src& src, int r = 0) {
   using Tr = MT<MODEL>;
   constexpr int P = Tr::P;
-  constexpr int CB = (N + L - 1) / L;
+  constexpr int CB = TAIL ? N / L : (N + L - 1) / L;
   constexpr bool SIG = Tr::EXTRA != 0;
   TF thf[P];
 #pragma unroll
@@ -789,7 +919,7 @@ __device__ __forceinline__ FinalOut<TF, MT<MODEL>::P> final_pass(const float* th
     const Pix<TF> fx = pix_from_edges<TF, SIG>(kf, flo, fhi);
     flo = fhi;
     const TF px = fx.phi;
-    const bool cvalid = (L == 1) || (c0 + cc < N);
+    const bool cvalid = (L == 1) || TAIL || (c0 + cc < N);
     TF ax = TF(0);
     if constexpr (SIG) ax = (MODEL == ASTIG) ? wf.dsx * fx.S : fx.S;
     float dcol[N], vcol[CAM == SCMOS ? N : 1];
@@ -814,6 +944,25 @@ __device__ __forceinline__ FinalOut<TF, MT<MODEL>::P> final_pass(const float* th
     // in flight.
 #pragma unroll 1
     for (int cc = 0; cc < CB; ++cc) column(cc);
+  }
+  if constexpr (TAIL) {
+    // this lane's rows r + L s of the last column, tables from their edges
+    static_assert(Src::kInRegisters, "tail pixels are register-resident");
+    const Pix<TF> fx = pix_from_edges<TF, SIG>(kf, edge_eval<TF, false>(kf, N - 1), edge_eval<TF, false>(kf, N));
+    const AxisK<TF> ky = axis_consts<TF>(thf[Tr::IY], wf.sy, nphf);
+    TF ax = TF(0);
+    if constexpr (SIG) ax = (MODEL == ASTIG) ? wf.dsx * fx.S : fx.S;
+#pragma unroll
+    for (int s = 0; s < row_slots<N, L>(); ++s) {
+      const int ri = r + L * s;
+      const Pix<TF> ry = pix_from_edges<TF, SIG>(ky, edge_eval<TF, false>(ky, ri), edge_eval<TF, false>(ky, ri + 1));
+      TF ay = TF(0);
+      if constexpr (SIG) ay = (MODEL == ASTIG) ? wf.dsy * ry.S : ry.S;
+      TF du[P];
+      final_du<MODEL, TF>(fx.phi, ry.phi, fx.D, ry.D, ax, ay, du);
+      const TF mu = bgf + nphf * fx.phi * ry.phi;
+      final_pixel<CAM, P, TF>((TF)src.dt[s], CAM == SCMOS ? (TF)src.vt[s] : TF(0), mu, du, ri < N, Fu, llr);
+    }
   }
   return final_reduce<L, P, TF>(Fu, llr);
 }
@@ -920,13 +1069,17 @@ template <int MODEL, int CAM, int N, int L, int PREC>
 __device__ __forceinline__ void fit_one(const FitArgs& a, const RoiRef& ref, bool active, int r) {
   using Tr = MT<MODEL>;
   constexpr int P = Tr::P;
-  constexpr int CB = (N + L - 1) / L;   // columns per lane
+  constexpr bool TAIL = tail_split<MODEL, CAM, N, L>();
+  constexpr int CB = block_cols<MODEL, CAM, N, L>();   // full columns per lane
+  constexpr int RS = row_slots<N, L>();                 // tail split: rows of the last column per lane
   constexpr bool SIG = Tr::EXTRA != 0;  // sigma / z terms present
   const int c0 = r * CB;
 
   // ---- load this lane's column block (and variances) once ----------------
   float dv[CB * N];
   float vr[CAM == SCMOS ? CB * N : 1];
+  float dt[TAIL ? RS : 1];
+  float vt[(TAIL && CAM == SCMOS) ? RS : 1];
   const float* src = ref.data;
   float dmax_l = 0.f;
   int cxc = 1, cyc = 1;
@@ -942,7 +1095,7 @@ __device__ __forceinline__ void fit_one(const FitArgs& a, const RoiRef& ref, boo
 #pragma unroll
   for (int cc = 0; cc < CB; ++cc) {
     const int col = c0 + cc;
-    const bool valid = (L == 1) || (col < N);
+    const bool valid = (L == 1) || TAIL || (col < N);
     if (valid) {
       load_run<N>(src + col * N, dv + cc * N);
     } else {
@@ -960,6 +1113,27 @@ __device__ __forceinline__ void fit_one(const FitArgs& a, const RoiRef& ref, boo
 #pragma unroll
     for (int i = 0; i < N; ++i) dmax_l = fmaxf(dmax_l, dv[cc * N + i]);
   }
+  if constexpr (TAIL) {
+    // rows r + L s of the last column; slots past the last row hold zeros
+    // (the row index is clamped, so no load leaves the ROI)
+#pragma unroll
+    for (int s = 0; s < RS; ++s) {
+      const int ri = r + L * s;
+      const bool ok = ri < N;
+      const int rc = ok ? ri : N - 1;
+      const float x = src[(N - 1) * N + rc];
+      dt[s] = ok ? x : 0.f;
+      if constexpr (CAM == SCMOS) {
+        if (inb) {
+          const float v = a.var_map[(int64_t)(cyc - 1 + rc) + (int64_t)a.ny * (int64_t)(cxc - 1 + N - 1)];
+          vt[s] = ok ? v : 0.f;
+        } else {
+          vt[s] = 0.f;
+        }
+      }
+      dmax_l = fmaxf(dmax_l, dt[s]);
+    }
+  }
   const float dmax = group_max<L>(dmax_l);
 
   // ---- initial estimates (unified_kernel.jl:158-255) ---------------------
@@ -975,12 +1149,16 @@ __device__ __forceinline__ void fit_one(const FitArgs& a, const RoiRef& ref, boo
         if (edge) esum += dv[cc * N + i];   // invalid columns hold zeros
       }
     }
+    if constexpr (TAIL) {
+#pragma unroll
+      for (int s = 0; s < RS; ++s) esum += dt[s];   // the last column is an edge column
+    }
     esum = group_sum<L>(esum);
     const float bg0 = esum / (float)(4 * N - 4);
     float tot = 0.f, xw = 0.f, yw = 0.f;
 #pragma unroll
     for (int cc = 0; cc < CB; ++cc) {
-      const bool valid = (L == 1) || (c0 + cc < N);
+      const bool valid = (L == 1) || TAIL || (c0 + cc < N);
       const float fj = (float)(c0 + cc + 1);
 #pragma unroll
       for (int i = 0; i < N; ++i) {
@@ -988,6 +1166,16 @@ __device__ __forceinline__ void fit_one(const FitArgs& a, const RoiRef& ref, boo
         tot += s;
         xw += s * fj;
         yw += s * (float)(i + 1);
+      }
+    }
+    if constexpr (TAIL) {
+#pragma unroll
+      for (int s = 0; s < RS; ++s) {
+        const int ri = r + L * s;
+        const float v = ri < N ? fmaxf(0.f, dt[s] - bg0) : 0.f;
+        tot += v;
+        xw += v * (float)N;
+        yw += v * (float)(ri + 1);
       }
     }
     tot = group_sum<L>(tot);
@@ -1009,7 +1197,7 @@ __device__ __forceinline__ void fit_one(const FitArgs& a, const RoiRef& ref, boo
       float mxx = 0.f, myy = 0.f;
 #pragma unroll
       for (int cc = 0; cc < CB; ++cc) {
-        const bool valid = (L == 1) || (c0 + cc < N);
+        const bool valid = (L == 1) || TAIL || (c0 + cc < N);
         const float dxp = (float)(c0 + cc + 1) - x0;
 #pragma unroll
         for (int i = 0; i < N; ++i) {
@@ -1017,6 +1205,17 @@ __device__ __forceinline__ void fit_one(const FitArgs& a, const RoiRef& ref, boo
           const float dyp = (float)(i + 1) - y0;
           mxx += s * dxp * dxp;
           myy += s * dyp * dyp;
+        }
+      }
+      if constexpr (TAIL) {
+        const float dxp = (float)N - x0;
+#pragma unroll
+        for (int s = 0; s < RS; ++s) {
+          const int ri = r + L * s;
+          const float v = ri < N ? fmaxf(0.f, dt[s] - bg0) : 0.f;
+          const float dyp = (float)(ri + 1) - y0;
+          mxx += v * dxp * dxp;
+          myy += v * dyp * dyp;
         }
       }
       mxx = group_sum<L>(mxx);
@@ -1060,18 +1259,47 @@ __device__ __forceinline__ void fit_one(const FitArgs& a, const RoiRef& ref, boo
     if constexpr (row_outer<MODEL, CAM, N, L>()) {
       const AxisK<float> kc = axis_consts<float>(th[Tr::IX], w.sx, nph);
       if (fast) {
-        accumulate_rows<MODEL, CAM, N, L, true, SHARE>(dv, vr, c0, nph, bg, kr, r, kc, w, G, H);
+        accumulate_rows<MODEL, CAM, N, L, true, SHARE>(dv, vr, dt, vt, c0, nph, bg, kr, r, kc, w, G, H);
       } else {
-        accumulate_rows<MODEL, CAM, N, L, false, SHARE>(dv, vr, c0, nph, bg, kr, r, kc, w, G, H);
+        accumulate_rows<MODEL, CAM, N, L, false, SHARE>(dv, vr, dt, vt, c0, nph, bg, kr, r, kc, w, G, H);
       }
     } else {
       AxisTab<float, N> ry;
+      const AxisK<float> kc = axis_consts<float>(th[Tr::IX], w.sx, nph);
+      // tail split: the last column's rows r + L s, with this lane's own row
+      // tables, folded here into Gt / Ht
+      float Gt[P], Ht[P];
+#pragma unroll
+      for (int k = 0; k < P; ++k) { Gt[k] = 0.f; Ht[k] = 0.f; }
       if constexpr (SHARE) {
-        shared_axis_tables<N, L, SIG>(kr, r, ry);
+        Pix<float> own[RS];
+        shared_axis_tables<N, L, SIG>(kr, r, ry, own);
+        if constexpr (TAIL) {
+          const Pix<float> tl = pix_from_edges<float, SIG>(kc, edge_eval<float, SIG>(kc, N - 1),
+                                                           edge_eval<float, SIG>(kc, N));
+          const float ncx_t = nph * tl.phi;
+          float St[11], rs[6];
+#pragma unroll
+          for (int q = 0; q < 11; ++q) St[q] = 0.f;
+#pragma unroll
+          for (int q = 0; q < 6; ++q) rs[q] = 0.f;
+          if (fast) {
+#pragma unroll
+            for (int s = 0; s < RS; ++s)
+              tail_pixel<MODEL, CAM, true>(own[s], w, dt[s], CAM == SCMOS ? vt[s] : 0.f, r + L * s < N, ncx_t, bg,
+                                           St, rs);
+            tail_fold<MODEL, CAM, true>(St, rs, tl, w, Gt, Ht);
+          } else {
+#pragma unroll
+            for (int s = 0; s < RS; ++s)
+              tail_pixel<MODEL, CAM, false>(own[s], w, dt[s], CAM == SCMOS ? vt[s] : 0.f, r + L * s < N, ncx_t, bg,
+                                            St, rs);
+            tail_fold<MODEL, CAM, false>(St, rs, tl, w, Gt, Ht);
+          }
+        }
       } else {
         axis_tables<float, N, SIG>(th[Tr::IY], w.sy, nph, 0, ry);
       }
-      const AxisK<float> kc = axis_consts<float>(th[Tr::IX], w.sx, nph);
 
       // Row extras: A = d(model)/d(extra) on the row axis, AA its second derivative.
       float Ay[SIG ? N : 1], AAy[SIG ? N : 1];
@@ -1105,6 +1333,10 @@ __device__ __forceinline__ void fit_one(const FitArgs& a, const RoiRef& ref, boo
       } else {
         accumulate<MODEL, CAM, N, L, false>(dv, vr, c0, nph, bg, ry, kc, w, Ay, AAy, sphi, sD, sD2, sA, sAA, G, H);
       }
+      if constexpr (TAIL) {
+#pragma unroll
+        for (int k = 0; k < P; ++k) { G[k] += Gt[k]; H[k] += Ht[k]; }
+      }
     }
 #pragma unroll
     for (int k = 0; k < P; ++k) {
@@ -1126,12 +1358,12 @@ __device__ __forceinline__ void fit_one(const FitArgs& a, const RoiRef& ref, boo
   // (final_pass_kernel) so that its registers do not lower this kernel's
   // occupancy.
   if constexpr (PREC == PREC_FP32) {
-    const RegSrc<N, CAM == SCMOS> srcr{dv, vr};
+    const RegSrc<N, CAM == SCMOS> srcr{dv, vr, dt, vt};
     FinalOut<float, P> fo;
     if constexpr (N > kMaxSmallBox) {
       fo = final_pass_rows<MODEL, CAM, N, L, float>(th, a.psf, c0, srcr);
     } else {
-      fo = final_pass<MODEL, CAM, N, L, float>(th, a.psf, c0, srcr);
+      fo = final_pass<MODEL, CAM, N, L, float, RegSrc<N, CAM == SCMOS>, TAIL>(th, a.psf, c0, srcr, r);
     }
     if (!inb) nan_outputs(th, fo);
     if (active && r == 0) store_outputs<P>(a, ref.out, ref.stride, th, fo);
