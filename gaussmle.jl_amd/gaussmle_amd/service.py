@@ -144,10 +144,10 @@ class Service:
         return {"slot_rois": a.value, "slot_pixels": b.value, "slots": c.value, "batches": d.value}
 
     def stats(self) -> dict:
-        v = np.zeros(9)
-        _lib.check(_lib.load().gmle_service_stats(self.handle, v.ctypes.data_as(_lib._DP), 9))
+        v = np.zeros(10)
+        _lib.check(_lib.load().gmle_service_stats(self.handle, v.ctypes.data_as(_lib._DP), 10))
         keys = ("batches", "rois_per_batch", "requests_per_batch", "host_launch_us", "gpu_us", "launch_to_seen_us",
-                "launch_to_start_us", "gpu_max_us", "launch_to_seen_max_us")
+                "launch_to_start_us", "gpu_max_us", "launch_to_seen_max_us", "batches_staged")
         return {k: round(float(x), 2) for k, x in zip(keys, v)}
 
     def close(self):

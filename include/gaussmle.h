@@ -365,8 +365,12 @@ int gmle_service_info(gmle_service* s, int64_t* slot_rois, int64_t* slot_pixels,
  * requests per batch, host time to form and launch a batch (us), GPU time of
  * a batch (us), time from launch to the completion being seen (us), time from
  * launch to the batch's start on the GPU being seen (us), largest GPU time of
- * a batch (us), largest launch-to-completion time (us).  A batch stops
- * growing at 65,536 ROIs (a larger request forms a batch of its own). */
+ * a batch (us), largest launch-to-completion time (us), batches whose raw
+ * counts were staged (packed on the host and copied by one DMA, with
+ * GMLE_SERVICE_INGEST=staged, or =auto for batches of at least 1 MiB of
+ * counts; by default the ingest kernel reads them in place).  A
+ * batch stops growing at 65,536 ROIs (a larger request forms a batch of its
+ * own). */
 int gmle_service_stats(gmle_service* s, double* out, int n);
 
 /* Waits for in-flight work, then frees everything. */

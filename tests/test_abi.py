@@ -35,6 +35,20 @@ def test_library_exports_every_declared_symbol(hiplib):
         assert name in _lib.SIGNATURES, f"{name} has no ctypes signature"
 
 
+def test_library_needs_no_recent_hip_symbol_versions(hiplib):
+    """The library binds to the HIP runtime torch bundles (loaded first, so
+    one runtime serves the process), which is older than the image's ROCm.
+    An entry point that exists only in a newer runtime (hipMemcpyBatchAsync
+    carries hip_7.1) makes the library fail to load there, so every HIP
+    symbol version it needs must be an old one."""
+    from gaussmle_amd import _lib
+    out = subprocess.run(["readelf", "-V", str(_lib.LIB_PATH)], capture_output=True, text=True, check=True).stdout
+    needs = out[out.index("Version needs"):] if "Version needs" in out else ""
+    versions = {tuple(int(x) for x in v.split(".")) for v in re.findall(r"Name: hip_([0-9.]+)", needs)}
+    assert versions, "no HIP symbol versions found"
+    assert max(versions) < (6, 0), sorted(versions)
+
+
 def test_scalar_queries(hiplib):
     assert hiplib.gmle_version().decode().startswith("gaussmle-hip")
     assert [hiplib.gmle_n_params(m) for m in range(4)] == [4, 5, 6, 5]

@@ -148,3 +148,32 @@ def test_service_mixed_precision_matches_batch_path():
         for r in reqs:
             r.close()
         s.close()
+
+
+@pytest.mark.parametrize("mode", ["direct", "staged", "dma", "auto"])
+def test_service_ingest_modes_give_identical_results(mode, monkeypatch):
+    """Raw counts read in place by the ingest kernel, packed on the host into
+    the slot's staging buffer and copied by one DMA, copied per request, or
+    chosen per batch by size: the pixels the fit sees are the same, so the
+    results are too."""
+    maps, frs = frames()
+    monkeypatch.setenv("GMLE_SERVICE_INGEST", mode)
+    psfs, cons, table = sv.default_setup(lanes=LANE_TABLES["lanes_1_2_4"])
+    s = sv.Service(psfs, cons, table, hbm_budget=1 << 30, slots=2)
+    reqs = [sv.Request(f) for f in frs]
+    try:
+        s.calibrate(*maps)
+        want = split(reference(s, frs), frs)
+        for t in [s.submit(r) for r in reqs]:
+            s.wait(t)
+        for i, r in enumerate(reqs):
+            assert_same(r.results(), want[i], f"{mode} request {i}")
+        staged = s.stats()["batches_staged"]
+        if mode == "staged":
+            assert staged > 0
+        elif mode != "auto":
+            assert staged == 0, (mode, staged)
+    finally:
+        for r in reqs:
+            r.close()
+        s.close()
