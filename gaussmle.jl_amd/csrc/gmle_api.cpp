@@ -223,6 +223,24 @@ struct OutPiece {
   void* host;
   size_t bytes;
 };
+// Offsets of the pieces in the pinned output buffer (256-byte aligned), and
+// the buffer grown to hold them.
+hipError_t out_layout(Cache& C, const OutPiece* pieces, int n, size_t* offs) {
+  size_t total = 0;
+  for (int k = 0; k < n; ++k) {
+    offs[k] = total;
+    total += (pieces[k].bytes + 255) & ~(size_t)255;
+  }
+  return C.pipe.ensure_out(total);
+}
+// After the stream's downloads into the pinned buffer: the caller's arrays.
+int out_finish(Cache& C, hipStream_t s, const OutPiece* pieces, int n, const size_t* offs) {
+  const hipError_t e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync (download)");
+  for (int k = 0; k < n; ++k) copy_parallel(pieces[k].host, (char*)C.pipe.out + offs[k], pieces[k].bytes);
+  return GMLE_OK;
+}
+
 int download(Cache& C, hipStream_t s, const OutPiece* pieces, int n) {
   size_t total = 0;
   for (int k = 0; k < n; ++k) total += (pieces[k].bytes + 255) & ~(size_t)255;
@@ -249,10 +267,13 @@ template <class EndFn>
 std::vector<size_t> chunk_cuts(int64_t n_rois, EndFn end) {
   std::vector<size_t> cuts;
   const size_t cap = kStageBytes / sizeof(float);
+  // the first chunks are smaller (1/8, 1/4, 1/2 of a buffer), so the first
+  // upload starts early and the copies, uploads and fits fill in sooner
+  auto cap_of = [&](size_t i) { return i < 3 ? cap >> (3 - i) : cap; };
   size_t start = 0;
   for (int64_t k = 0; k < n_rois; ++k) {
     const size_t e = end(k);
-    if (e - start > cap && k > 0 && end(k - 1) > start) {
+    if (e - start > cap_of(cuts.size()) && k > 0 && end(k - 1) > start) {
       cuts.push_back(end(k - 1));
       start = end(k - 1);
     }
@@ -702,6 +723,17 @@ int gmle_fit(int model, const float* psf, int box, int64_t n_rois, const float* 
   base.llr = (float*)C.buf[3];
   base.crlb64 = want64 ? (double*)C.buf[6] : nullptr;
   base.llr64 = want64 ? (double*)C.buf[7] : nullptr;
+  // outputs: each chunk's results go to the pinned output buffer right after
+  // its fit (downloads overlap the later chunks' uploads); the caller's arrays
+  // are written once every chunk has passed the check
+  OutPiece pieces[5] = {{base.params, params, (size_t)n_rois * P * sizeof(float)},
+                        {base.crlb, crlb, (size_t)n_rois * P * sizeof(float)},
+                        {base.llr, llr, (size_t)n_rois * sizeof(float)}};
+  int np = 3;
+  if (want64 && crlb64) pieces[np++] = {base.crlb64, crlb64, (size_t)n_rois * P * sizeof(double)};
+  if (want64 && llr64) pieces[np++] = {base.llr64, llr64, (size_t)n_rois * sizeof(double)};
+  size_t offs[5];
+  if ((e = out_layout(C, pieces, np, offs)) != hipSuccess) return hip_fail(e, "hipHostMalloc (output staging)");
   // pixels stream in through the pinned ring; each chunk is fitted as soon as
   // it is on the device, with the lanes chosen for the whole call (a ROI's
   // result does not depend on which launch holds it)
@@ -720,21 +752,21 @@ int gmle_fit(int model, const float* psf, int box, int64_t n_rois, const float* 
     a.llr += r0;
     if (a.crlb64) a.crlb64 += r0 * P;
     if (a.llr64) a.llr64 += r0;
-    const hipError_t le = fn(a, s);
-    return le == hipSuccess ? GMLE_OK : hip_fail(le, "kernel launch");
+    hipError_t le = fn(a, s);
+    if (le != hipSuccess) return hip_fail(le, "kernel launch");
+    for (int q = 0; q < np && le == hipSuccess; ++q) {
+      const size_t per = pieces[q].bytes / (size_t)n_rois;   // bytes per ROI
+      le = hipMemcpyAsync((char*)C.pipe.out + offs[q] + (size_t)r0 * per, (const char*)pieces[q].dev + r0 * per,
+                          (size_t)(r1 - r0) * per, hipMemcpyDeviceToHost, s);
+    }
+    return le == hipSuccess ? GMLE_OK : hip_fail(le, "hipMemcpyAsync (download)");
   };
   rc = pipe_upload(C.pipe, data, d_data, cuts.data(), (int)cuts.size(), s, launch);
   if (rc != GMLE_OK) {
     (void)hipStreamSynchronize(s);
     return rc;
   }
-  OutPiece pieces[5] = {{base.params, params, (size_t)n_rois * P * sizeof(float)},
-                        {base.crlb, crlb, (size_t)n_rois * P * sizeof(float)},
-                        {base.llr, llr, (size_t)n_rois * sizeof(float)}};
-  int np = 3;
-  if (want64 && crlb64) pieces[np++] = {base.crlb64, crlb64, (size_t)n_rois * P * sizeof(double)};
-  if (want64 && llr64) pieces[np++] = {base.llr64, llr64, (size_t)n_rois * sizeof(double)};
-  return download(C, s, pieces, np);
+  return out_finish(C, s, pieces, np, offs);
 }
 
 int gmle_shape_index(int model, int box) { return shape_index(model, box); }
