@@ -2,15 +2,19 @@
 //
 // The separable model needs, at every pixel edge, erf and the Gaussian
 // exp(-u^2) (u = t / (sigma sqrt 2)); the Gaussian is needed anyway for the
-// derivative tables, so the erf side costs one polynomial and no
-// transcendental of its own.  The kernels work in the scaled variable
+// derivative tables, so the erf side costs one reciprocal and one polynomial.
+// The kernels work in the scaled variable
 //   v = k u,  k = sqrt(log2 e),  so that  exp(-u^2) = exp2(-v^2)
 // (one multiply and one v_exp_f32 per edge), and evaluate
-//   erfc(|u|) = exp2(-v^2) R(min(|v|, kErfVMax) - kErfC),   R(v) ~ erfcx(v / k)
-// with R a near-minimax polynomial of degree 16 fitted by gen_erf_coeffs.py
-// (Lawson-reweighted least squares on Chebyshev nodes) over |u| <= 5.6, where
-// erfc(5.6) = 1.5e-15 lies below any tail a fit can see.  Measured Float32
-// FMA-Horner error: 3.6e-7 relative on erfcx.
+//   erfc(|u|) = exp2(-v^2) R(t),   t = 1 - 2 c / (|v| + c),   R(t) ~ erfcx(|u|)
+// The map t takes |v| in [0, inf) onto [-1, 1) and turns erfcx's slow 1/u
+// tail into a smooth function of t, so a degree-10 polynomial R (fitted by
+// gen_erf_coeffs.py: Lawson-reweighted least squares on Chebyshev nodes, over
+// |u| <= 10, beyond which exp2(-v^2) underflows) has the accuracy a degree-16
+// polynomial in |v| had.  Measured Float32 error of R: 3.7e-7 relative with a
+// correctly rounded reciprocal, 5.8e-7 with the reciprocal one ulp off (what
+// v_rcp_f32 may return).  Per edge: one add, one v_rcp_f32, one FMA for t and
+// ten FMAs, against a clamp, a subtraction and sixteen FMAs before.
 //
 // Pixel integrals.  (erf(b) - erf(a)) / 2 of a pixel in the tails subtracts
 // two values close to +-1 and keeps only their rounding: an absolute error of
@@ -20,9 +24,8 @@
 // erf(u) = 2 hs - se, and form
 //   Phi = (erf(b) - erf(a)) / 2 = (se_a - se_b) / 2 + (hs_b - hs_a)
 // where the second term is 0, or 1 for the pixel that straddles the centre.
-// Every pixel integral then carries R's relative accuracy.  Beyond |u| = 5.6
-// R is held at its end value (erfc overestimated by 16 % at u = 6.5, where
-// erfc < 1e-19).  Callable from host code too, for testing.
+// Every pixel integral then carries R's relative accuracy.  Callable from
+// host code too, for testing (the reciprocal is then correctly rounded).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -30,30 +33,34 @@
 
 namespace gmle {
 
-constexpr float kErfK = 1.2011224031448364f;     // sqrt(log2 e): v = k u
-constexpr float kErfVMax = 6.726285457611084f;   // k * 5.6
-constexpr float kErfC = 3.363142728805542f;      // kErfVMax / 2
+constexpr float kErfK = 1.2011224031448364f;   // sqrt(log2 e): v = k u
+constexpr float kErfC = 3.0028059482574463f;   // c = k * 2.5: t = 0 at |u| = 2.5
+constexpr float kErfC2 = 6.005611896514893f;   // 2 c (exact)
 
-// R(min(|v|, kErfVMax) - kErfC) ~ erfcx(|v| / k).
+// 1/x: v_rcp_f32 on the device (1 ulp), a division in host code.
+__host__ __device__ __forceinline__ float erf_rcp(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rcpf(x);
+#else
+  return 1.0f / x;
+#endif
+}
+
+// R(t), t = 1 - 2c / (|v| + c): erfcx(|v| / k).  v = +-inf gives t = 1
+// (finite), NaN propagates.
 __host__ __device__ __forceinline__ float erfcx_poly(float v) {
-  const float x = fminf(fabsf(v), kErfVMax) - kErfC;
-  float r = 1.5094743163746038e-11f;
-  r = fmaf(r, x, -9.99696297965258e-11f);
-  r = fmaf(r, x, -1.3795589670628772e-10f);
-  r = fmaf(r, x, 1.419491746368351e-09f);
-  r = fmaf(r, x, 5.186261198275588e-09f);
-  r = fmaf(r, x, -3.8145707037529064e-08f);
-  r = fmaf(r, x, 6.890451231811312e-08f);
-  r = fmaf(r, x, -2.4120461716847785e-07f);
-  r = fmaf(r, x, 1.846569261942932e-06f);
-  r = fmaf(r, x, -9.162412425212096e-06f);
-  r = fmaf(r, x, 3.9917653339216486e-05f);
-  r = fmaf(r, x, -0.00017699696763884276f);
-  r = fmaf(r, x, 0.0007703695446252823f);
-  r = fmaf(r, x, -0.003233563620597124f);
-  r = fmaf(r, x, 0.013096202164888382f);
-  r = fmaf(r, x, -0.051039982587099075f);
-  r = fmaf(r, x, 0.1905488669872284f);
+  const float t = fmaf(-kErfC2, erf_rcp(fabsf(v) + kErfC), 1.0f);
+  float r = -3.776571247726679e-05f;
+  r = fmaf(r, t, -0.00011116439418401569f);
+  r = fmaf(r, t, 0.00023663679894525558f);
+  r = fmaf(r, t, 0.0008690135437063873f);
+  r = fmaf(r, t, -0.0026459149084985256f);
+  r = fmaf(r, t, -0.0039923484437167645f);
+  r = fmaf(r, t, 0.039923105388879776f);
+  r = fmaf(r, t, -0.1250331848859787f);
+  r = fmaf(r, t, 0.2517131567001343f);
+  r = fmaf(r, t, -0.3717367351055145f);
+  r = fmaf(r, t, 0.21080636978149414f);
   return r;
 }
 
