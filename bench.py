@@ -814,6 +814,10 @@ def main(argv=None) -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": round(value / REFERENCE_GPU_FITS_PER_S, 3) if args.workload == "xynb11" else None,
+        "vs_baseline_note": ("value (inputs resident in HBM) over the reference's published ~1e7 fits/s for 11x11 "
+                             "XYNB on an RTX 4090 through fit() on host arrays; the host-array call of this "
+                             "library against the same figure is host_path.vs_reference_gpu_host_call")
+        if args.workload == "xynb11" else None,
         "dtype": "f32" if args.precision == "fp32" else "f32+f64",
         "data": "synthetic: seeded Poisson ROIs (numpy PCG64), device-resident before timing",
         "config": {"workload": spec["label"], "model": type(w.psf_pixels).__name__, "box": w.box,
