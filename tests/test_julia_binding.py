@@ -164,8 +164,8 @@ def test_psf_type_methods_match_the_python_binding():
 
 
 def test_integration_snippet_calls_what_exists():
-    """The GPU branch INTEGRATION.md shows calls hip_fit! with the PSF struct
-    and constraints, a method the shim defines."""
+    """The GPU branch the integration notes at the repository root show calls
+    hip_fit! with the PSF struct and constraints, a method the shim defines."""
     doc = (ROOT / "INTEGRATION.md").read_text()
     snippet = re.search(r"```julia\n(.*?)```", doc, re.S).group(1)
     assert "GaussMLEHIP.hip_fit!(" in snippet
