@@ -174,6 +174,9 @@ CONFIGS = {
                     iters=20, camera="scmos", rois=1_000_000, gen="config1_device"),
     "config2": dict(label="AstigmaticXYZNB 15x15, Poisson, 30 iterations", model="astig", box=15, iters=30,
                     camera="ideal", rois=4_000_000, gen="config2_device"),
+    # boxes above 16 px (8 lanes per ROI)
+    "box21": dict(label="GaussianXYNBS 21x21 (sigma 1.5-2.5 px), Poisson, 20 iterations", model="xynbs", box=21,
+                  iters=20, camera="ideal", rois=1_000_000, gen="large_box_device"),
 }
 
 
@@ -652,7 +655,7 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes (HBM counter calibration)")
-    ap.add_argument("--configs", default="config1,config2",
+    ap.add_argument("--configs", default="config1,config2,box21",
                     help="configurations reported beside the headline line (comma list of "
                          f"{','.join(CONFIGS)}; empty: none)")
     ap.add_argument("--trace-summary", default=None, metavar="CSV",

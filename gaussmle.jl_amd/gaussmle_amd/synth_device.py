@@ -127,6 +127,19 @@ def config2_device(n: int = 4_000_000, seed: int = 2, box: int = 15, iterations:
                           "ideal", truth=theta)
 
 
+def large_box_device(n: int = 1_000_000, seed: int = 5, box: int = 21, sigma_range=(1.5, 2.5),
+                     iterations: int = 20) -> DeviceWorkload:
+    """Large ROIs: GaussianXYNBS of box x box px (sigma uniform in sigma_range,
+    other draws as synth.py's), Poisson, with the reference's default
+    constraints for the ROI's own size; pixels simulated on the device."""
+    rng = np.random.default_rng(seed)
+    psf = GaussianXYNBS()
+    theta = _draw(1, n, box, rng, sigma_range)
+    data = _simulate(psf, box, theta, seed)
+    return DeviceWorkload(f"xynbs_ideal_{box}", psf, box, n, {"data": data}, default_constraints(psf, box),
+                          iterations, "ideal", truth=theta)
+
+
 def config3_device(n: int = 4_000_000, seed: int = 3, boxes=(7, 9, 11, 13, 15), box_weights=(0.1, 0.2, 0.4, 0.2, 0.1),
                    models=(0, 1, 2), cam: int = 256, sigma_range=(1.0, 1.8), iterations: int = 20):
     """The uneven batch of batch.config3 (box sizes and models as there, sCMOS
