@@ -199,7 +199,7 @@ def time_launches(fn, steps: int, warmup: int):
     return wall, float(np.mean([a.elapsed_time(b) for a, b in ev])) / 1e3
 
 
-def run_config(key: str, steps: int, warmup: int, n: int | None = None) -> dict:
+def run_config(key: str, steps: int, warmup: int, n: int | None = None, lanes_override: int = 0) -> dict:
     """One of CONFIGS at its full size, in Float32 (one fit kernel) and in
     mixed precision (fit kernel + Float64 final pass), with the roofline
     position of each (algorithmic FLOPs as for the headline)."""
@@ -219,8 +219,8 @@ def run_config(key: str, steps: int, warmup: int, n: int | None = None) -> dict:
     res = {"workload": spec["label"], "rois": dw.n, "data": "synthetic: seeded parameters (numpy PCG64), pixels "
            "drawn by the on-device simulator (Philox4x32-10), device-resident before timing"}
     for precision in ("fp32", "mixed"):
-        lanes = lib.gmle_default_lanes(dw.psf_pixels.model_id, 0 if dw.camera == "ideal" else 1, dw.box,
-                                       _lib.PRECISION_IDS[precision], dw.n)
+        lanes = lanes_override or lib.gmle_default_lanes(dw.psf_pixels.model_id, 0 if dw.camera == "ideal" else 1,
+                                                         dw.box, _lib.PRECISION_IDS[precision], dw.n)
         out = {}
 
         def step():
@@ -683,7 +683,7 @@ def main(argv=None) -> int:
     if args.workload == "mixed":
         return main_mixed(args)
     if args.workload in CONFIGS:
-        res = run_config(args.workload, args.steps, args.warmup, args.rois or None)
+        res = run_config(args.workload, args.steps, args.warmup, args.rois or None, args.lanes)
         print(json.dumps({"metric": METRIC, args.workload: res}), flush=True)
         return 0
     if args.workload == "boxer":
