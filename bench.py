@@ -34,9 +34,7 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "ROI fits/sec (11×11 XYNB, 20 iters) plain vs batched; p99 latency under load"
 PEAK_FP32_TFLOPS = 157.3
-PEAK_FP32_UNPACKED_TFLOPS = 78.6   # FP32 vector without packed (2-wide) instructions
 PEAK_HBM_GBS = 8000.0
-VALU_LANE_OPS_PER_S = 256 * 4 * 16 * 2.4e9   # CUs x SIMDs x lanes x clock: one FP32 op per lane-cycle
 # GPU fits/s quoted by the reference for 11x11 GaussianXYNB (RTX 4090, batch 50K).
 REFERENCE_GPU_FITS_PER_S = 1.0e7
 
@@ -566,6 +564,30 @@ def instruction_mix(workload: str):
         return None
 
 
+def issue_rates():
+    """Clocks per SIMD instruction at two resident waves per SIMD (the fit
+    kernels' occupancy), measured by gmle_issue_probe, if recorded."""
+    p = ROOT / "profiles" / "valu_issue_rates.json"
+    try:
+        return json.loads(p.read_text())["model_at_2_waves"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def issue_model(mix, rates, waves_per_simd: float, kernel_ms: float):
+    """Launch time the profiled instruction mix needs at the measured issue
+    rates and clock, and the kernel's fraction of it (1 = issue-bound with
+    no stall)."""
+    if not mix or not rates or not mix.get("clock_ghz_under_load"):
+        return None
+    trans = mix["valu_trans_f32"]
+    cycles = (mix["valu"] - trans) * rates["cycles_per_valu"] + trans * rates["cycles_per_trans"]
+    ms = waves_per_simd * cycles / (mix["clock_ghz_under_load"] * 1e9) * 1e3
+    return {"ms": round(ms, 4), "frac": round(ms / kernel_ms, 4), "cycles_per_valu": rates["cycles_per_valu"],
+            "cycles_per_trans": rates["cycles_per_trans"], "clock_ghz": mix["clock_ghz_under_load"],
+            "waves_per_simd": round(waves_per_simd, 3)}
+
+
 def cpu_baseline(w, sample: int, threads: int, runs: int = 5):
     """Time the oracle (the reference's algorithm restated in C++, Float32) on
     the first `sample` ROIs with `threads` OpenMP threads: one warm-up run,
@@ -755,19 +777,18 @@ def main(argv=None) -> int:
     traffic = traffic_from_profiles(args.workload, n)
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
-                "frac_unpacked": round(achieved / PEAK_FP32_UNPACKED_TFLOPS, 4),
                 "kernel_ms": round(avg_kernel_s * 1e3, 4), "flops_per_fit": F,
                 "hbm_gbs": round(bytes_per_fit(w.box, len(w.psf_pixels)) * n / avg_kernel_s / 1e9, 2)}
     mix = instruction_mix(args.workload)
-    if mix:
-        # transcendental rate against the quarter-rate ceiling of the VALU lanes
-        trans = mix["valu_trans_f32"] * n / avg_kernel_s
-        roofline["trans_per_fit"] = mix["valu_trans_f32"]
-        roofline["trans_util"] = round(trans / (VALU_LANE_OPS_PER_S / 4), 4)
-        # Instructions per fit from the profile; the issue-slot share comes
-        # from measured GPU cycles there (GRBM_GUI_ACTIVE), not from a clock
-        # assumed here.
+    if mix and lanes == 1 and args.precision == "fp32":
+        # lane instructions per fit (= wave instructions per wave of 64
+        # one-lane fits), from the profile of this kernel
         roofline["valu_insts_per_fit"] = mix["valu"]
+        roofline["trans_per_fit"] = mix["valu_trans_f32"]
+        simds = torch.cuda.get_device_properties(dev).multi_processor_count * 4
+        model = issue_model(mix, issue_rates(), n / 64 / simds, avg_kernel_s * 1e3)
+        if model:
+            roofline["issue_model"] = model
 
     cpu = None
     if not args.no_cpu_baseline:

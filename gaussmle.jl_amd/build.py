@@ -22,6 +22,8 @@ CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIBDIR = PKG / "lib"
 LIB = LIBDIR / "libgaussmle_hip.so"
+PROBE_SRC = CSRC / "probe" / "issue_probe.hip"
+PROBE = LIBDIR / "gmle_issue_probe"
 INCLUDE = PKG.parent / "include"
 
 ARCH = "gfx950"
@@ -113,12 +115,28 @@ def build(force: bool = False, jobs: int | None = None, verbose: bool = True) ->
     return LIB
 
 
+def build_probe(force: bool = False, verbose: bool = True) -> Path:
+    """The VALU / MFMA issue-rate probe (a stand-alone executable, not part of
+    the library).  Its MFMA accumulators are kept in VGPRs so the mixed
+    MFMA + FMA loop carries no AGPR copies."""
+    if force or not PROBE.exists() or PROBE.stat().st_mtime < PROBE_SRC.stat().st_mtime:
+        LIBDIR.mkdir(exist_ok=True)
+        cmd = [hipcc(), *HIPFLAGS, "-mllvm", "-amdgpu-mfma-vgpr-form=1", str(PROBE_SRC), "-o", str(PROBE)]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"hipcc failed for {PROBE_SRC.name}:\n{r.stderr[-4000:]}")
+    if verbose:
+        print(f"[build] {PROBE}", flush=True)
+    return PROBE
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--force", action="store_true", help="recompile every translation unit")
     ap.add_argument("--jobs", type=int, default=None)
     args = ap.parse_args(argv)
     build(force=args.force, jobs=args.jobs)
+    build_probe(force=args.force)
     return 0
 
 
