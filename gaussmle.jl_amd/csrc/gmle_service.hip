@@ -21,12 +21,14 @@
 //   * staged: worker threads pack the batch's counts back to back into the
 //     slot's pinned staging buffer, and one DMA copies them into the arena at
 //     the copy engine's rate (~57 GB/s);
-//   * dma: one DMA per request straight from its buffer (per-copy overhead);
-//   * auto: staged for batches with at least kStagedMinBytes of counts,
-//     direct below.
-// Direct is the default: on 2048 x 2048 frames of 50-5000 ROIs it gave the
-// highest peak (9.2e7 fits/s against 8.7e7 for auto) and the lowest p99; the
-// others are chosen with GMLE_SERVICE_INGEST=staged|dma|auto.
+//   * dma: one DMA per run of requests whose counts lie back to back in host
+//     memory (requests whose buffers come from one pinned pool in submission
+//     order form one run per batch; separate buffers cost one copy each);
+//   * auto (the default): dma for runs of at least kDmaMinRunBytes, direct for
+//     the rest of the batch.
+// GMLE_SERVICE_INGEST=direct|staged|dma|auto overrides the default.  On
+// 2048 x 2048 frames of 50-5000 ROIs from one pool, with the queue kept full,
+// auto sustains ~1.4-1.5e8 fits/s against ~9e7 for direct.
 // The sCMOS conversion electrons = (adu - offset) * gain follows
 // src/camera_models.jl:110-113 with per-pixel maps gathered through the ROI
 // corner, the indexing the kernel uses for the variance map
@@ -66,8 +68,13 @@ constexpr int kMaxReqPerBatch = 4096;
 // slot's PCIe copies overlap another slot's fits.
 constexpr int64_t kTargetBatchRois = 1 << 16;
 constexpr int64_t kXferBlocks = 256;
-// Auto ingest: batches with at least this many bytes of raw counts are staged.
-constexpr int64_t kStagedMinBytes = (int64_t)1 << 20;
+// Grid cap of the ingest kernel when a batch's counts are already in the
+// arena (8 blocks per CU).
+constexpr int64_t kDeviceIngestBlocks = 2048;
+// Auto ingest: a run of requests whose raw counts lie back to back in host
+// memory moves by one DMA from this many bytes up (one copy costs ~10-15 us
+// of fixed overhead, so shorter runs are read in place).
+constexpr int64_t kDmaMinRunBytes = (int64_t)1 << 20;
 // Pieces of the host-side packing handed to one worker.
 constexpr size_t kPackPiece = (size_t)256 << 10;
 enum IngestMode { INGEST_AUTO = 0, INGEST_DIRECT = 1, INGEST_STAGED = 2, INGEST_DMA = 3 };
@@ -332,6 +339,10 @@ struct Slot {
   // pinned staging of the batch's raw counts (staged ingest), grown on demand
   uint16_t* stage = nullptr;
   size_t stage_bytes = 0;
+  // the batch's runs of requests with adjacent host counts ([first, end)) and
+  // which requests arrive by DMA (dma ingest, or auto for long runs)
+  std::vector<std::pair<size_t, size_t>> runs;
+  std::vector<uint8_t> req_dma;
   // pinned tables (host and device addresses)
   ReqDesc* reqs_h = nullptr;
   ReqDesc* reqs_d = nullptr;
@@ -378,8 +389,10 @@ struct gmle_service {
   double stat_batches = 0, stat_rois = 0, stat_reqs = 0, stat_host = 0, stat_gpu = 0, stat_seen = 0, stat_start = 0;
   double stat_gpu_max = 0, stat_seen_max = 0;
   int launch_error = GMLE_OK;
-  int ingest_mode = gmle::INGEST_DIRECT;   // how raw counts reach the GPU (IngestMode)
+  int ingest_mode = gmle::INGEST_AUTO;     // how raw counts reach the GPU (IngestMode)
   int64_t batches_staged = 0;            // batches whose counts went through the staging buffer
+  std::atomic<int64_t> raw_bytes{0}, raw_bytes_dma{0};   // raw counts ingested, and the part moved by DMA
+  std::atomic<int64_t> dma_copies{0};                    // host-to-device copies of raw counts
   int64_t target_rois = gmle::kTargetBatchRois;   // a batch closes once it holds this many ROIs
   int64_t xfer_blocks = gmle::kXferBlocks;        // grid of the ingest / result kernels
   std::string error_msg;        // first failure of a batch (launch or completion)
@@ -597,6 +610,35 @@ bool stage_counts(Slot& sl, int64_t pix) {
   return true;
 }
 
+// Splits the batch into runs of requests whose raw counts follow each other in
+// host memory (request r + 1's first count right after request r's last), and
+// marks the requests whose run moves by DMA: every run in dma mode, runs of at
+// least kDmaMinRunBytes in auto mode, none otherwise.  Requests are laid out
+// back to back in the arena in batch order, so a run is also contiguous there
+// and one copy moves it.  Returns the bytes marked for DMA.
+int64_t plan_runs(Slot& sl, int mode) {
+  const size_t n = sl.batch.size();
+  sl.runs.clear();
+  sl.req_dma.assign(n, 0);
+  int64_t dma = 0;
+  for (size_t r = 0; r < n;) {
+    const char* end = (const char*)sl.batch[r]->adu + (size_t)sl.batch[r]->npix * sizeof(uint16_t);
+    size_t r2 = r + 1;
+    while (r2 < n && (const char*)sl.batch[r2]->adu == end) {
+      end += (size_t)sl.batch[r2]->npix * sizeof(uint16_t);
+      ++r2;
+    }
+    const int64_t bytes = (int64_t)(end - (const char*)sl.batch[r]->adu);
+    if (mode == INGEST_DMA || (mode == INGEST_AUTO && bytes >= kDmaMinRunBytes)) {
+      sl.runs.emplace_back(r, r2);
+      for (size_t k = r; k < r2; ++k) sl.req_dma[k] = 1;
+      dma += bytes;
+    }
+    r = r2;
+  }
+  return dma;
+}
+
 int launch_slot(gmle_service* s, Slot& sl) {
   hipError_t e;
   int64_t roi = 0, pix = 0, nch = 0;
@@ -604,10 +646,12 @@ int launch_slot(gmle_service* s, Slot& sl) {
   int64_t batch_pix = 0;
   for (const auto& qp : sl.batch) batch_pix += qp->npix;
   int mode = s->ingest_mode;
-  if (mode == INGEST_AUTO)
-    mode = batch_pix * (int64_t)sizeof(uint16_t) >= kStagedMinBytes ? INGEST_STAGED : INGEST_DIRECT;
   if (mode == INGEST_STAGED && !stage_counts(sl, batch_pix)) mode = INGEST_DIRECT;
   if (mode == INGEST_STAGED) ++s->batches_staged;
+  const int64_t dma_bytes = plan_runs(sl, mode);
+  s->raw_bytes += batch_pix * (int64_t)sizeof(uint16_t);
+  s->raw_bytes_dma += mode == INGEST_STAGED ? batch_pix * (int64_t)sizeof(uint16_t) : dma_bytes;
+  s->dma_copies += mode == INGEST_STAGED ? 1 : (int64_t)sl.runs.size();
   for (size_t r = 0; r < sl.batch.size(); ++r) {
     const Pending& q = *sl.batch[r];
     ReqDesc& d = sl.reqs_h[r];
@@ -615,7 +659,7 @@ int launch_slot(gmle_service* s, Slot& sl) {
     d.boxes = q.boxes;
     d.xc = q.xc;
     d.yc = q.yc;
-    d.adu = mode == INGEST_DIRECT ? q.adu_dev : sl.raw + pix;
+    d.adu = (mode == INGEST_STAGED || sl.req_dma[r]) ? sl.raw + pix : q.adu_dev;
     d.params = q.params;
     d.crlb = q.crlb;
     d.llr = q.llr;
@@ -638,7 +682,12 @@ int launch_slot(gmle_service* s, Slot& sl) {
   ResultArgs ra{sl.reqs_d, sl.chunks_d, nch, sl.params, sl.crlb, sl.llr};
   // Host-memory transfers wait on PCIe: a bounded grid keeps enough requests
   // in flight without parking every CU, so other slots' fits run beside them.
+  // Counts already in the arena (staged, or every run by DMA) are converted
+  // from HBM by a grid of up to one block per chunk.
   const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>(nch, s->xfer_blocks));
+  const bool raw_on_device = mode == INGEST_STAGED || dma_bytes == batch_pix * (int64_t)sizeof(uint16_t);
+  const unsigned gi =
+      raw_on_device ? (unsigned)std::max<int64_t>(1, std::min<int64_t>(nch, kDeviceIngestBlocks)) : gx;
   BatchPlan bp = slot_plan(sl, roi);
   int64_t start = 0;
   for (int k = 0; k < kNumShapes; ++k) {
@@ -647,7 +696,7 @@ int launch_slot(gmle_service* s, Slot& sl) {
     start += counts[k];
   }
   const unsigned gb = (unsigned)((roi + kBatchBlock - 1) / kBatchBlock);
-  if ((e = set_kernel_node(sl.exec, sl.n_ingest, (const void*)&service_ingest, gx, kIngestBlock, &ia)) != hipSuccess ||
+  if ((e = set_kernel_node(sl.exec, sl.n_ingest, (const void*)&service_ingest, gi, kIngestBlock, &ia)) != hipSuccess ||
       (e = set_kernel_node(sl.exec, sl.n_init, batch_init_kernel(), gb, kBatchBlock, &bp)) != hipSuccess ||
       (e = set_kernel_node(sl.exec, sl.n_scatter, batch_scatter_kernel(), gb, kBatchBlock, &bp)) != hipSuccess ||
       (e = set_kernel_node(sl.exec, sl.n_results, (const void*)&service_results, gx, kIngestBlock, &ra)) != hipSuccess)
@@ -694,10 +743,12 @@ int launch_slot(gmle_service* s, Slot& sl) {
                             sl.stream)) != hipSuccess)
       return hip_fail(e, "hipMemcpyAsync (staged counts)");
   }
-  for (size_t r = 0; mode == INGEST_DMA && r < sl.batch.size(); ++r) {
-    const Pending& q = *sl.batch[r];
-    if ((e = hipMemcpyAsync(sl.raw + (sl.reqs_h[r].pix_base), q.adu, (size_t)q.npix * sizeof(uint16_t),
-                            hipMemcpyHostToDevice, sl.stream)) != hipSuccess)
+  for (const auto& run : sl.runs) {
+    const Pending& q0 = *sl.batch[run.first];
+    const int64_t p0 = sl.reqs_h[run.first].pix_base;
+    const int64_t p1 = sl.reqs_h[run.second - 1].pix_base + sl.batch[run.second - 1]->npix;
+    if ((e = hipMemcpyAsync(sl.raw + p0, q0.adu, (size_t)(p1 - p0) * sizeof(uint16_t), hipMemcpyHostToDevice,
+                            sl.stream)) != hipSuccess)
       return hip_fail(e, "hipMemcpyAsync (request counts)");
   }
   if ((e = hipGraphLaunch(sl.exec, sl.stream)) != hipSuccess) return hip_fail(e, "hipGraphLaunch");
@@ -986,12 +1037,12 @@ int gmle_service_create(int device, size_t hbm_budget, int slots, int precision,
   std::memcpy(s->lanes, lanes, sizeof(s->lanes));
   s->budget = hbm_budget;
   s->max_box = max_box;
-  // How raw counts reach the GPU (ingest modes above): direct unless
+  // How raw counts reach the GPU (ingest modes above): auto unless
   // GMLE_SERVICE_INGEST names another.
   if (const char* ingest = std::getenv("GMLE_SERVICE_INGEST")) {
     const std::string m(ingest);
-    s->ingest_mode = m == "staged" ? INGEST_STAGED : m == "dma" ? INGEST_DMA : m == "auto" ? INGEST_AUTO
-                                                                                          : INGEST_DIRECT;
+    s->ingest_mode = m == "staged" ? INGEST_STAGED : m == "dma" ? INGEST_DMA : m == "direct" ? INGEST_DIRECT
+                                                                                            : INGEST_AUTO;
   }
   // Batch size target (ROIs): smaller batches let one slot's ingest overlap
   // another's fit; larger ones amortize the per-batch latency floor.
@@ -1135,10 +1186,12 @@ int gmle_service_stats(gmle_service* s, double* out, int n) {
   if (!s || !out) return fail(GMLE_ERR_ARG, "NULL argument");
   std::lock_guard<std::mutex> lock(s->mu);
   const double b = s->stat_batches > 0 ? s->stat_batches : 1.0;
-  const double v[10] = {s->stat_batches, s->stat_rois / b, s->stat_reqs / b, s->stat_host / b * 1e6,
+  const double raw = (double)s->raw_bytes.load();
+  const double v[12] = {s->stat_batches, s->stat_rois / b, s->stat_reqs / b, s->stat_host / b * 1e6,
                         s->stat_gpu / b * 1e6, s->stat_seen / b * 1e6, s->stat_start / b * 1e6,
-                        s->stat_gpu_max * 1e6, s->stat_seen_max * 1e6, (double)s->batches_staged};
-  for (int k = 0; k < n && k < 10; ++k) out[k] = v[k];
+                        s->stat_gpu_max * 1e6, s->stat_seen_max * 1e6, (double)s->batches_staged,
+                        raw > 0 ? (double)s->raw_bytes_dma.load() / raw : 0.0, (double)s->dma_copies.load() / b};
+  for (int k = 0; k < n && k < 12; ++k) out[k] = v[k];
   return GMLE_OK;
 }
 

@@ -41,6 +41,31 @@ class PinnedArray:
             self.array = None
 
 
+class CountPool:
+    """One pinned allocation that hands out raw-count buffers back to back.
+
+    Requests whose counts come from one pool in submission order lie next to
+    each other in host memory, so the service moves a batch of them to the
+    GPU with one DMA (dma ingest, or auto for runs of 1 MiB and more) instead
+    of reading them in place."""
+
+    def __init__(self, n_counts: int):
+        self._buf = PinnedArray((max(int(n_counts), 1),), np.uint16)
+        self._at = 0
+
+    def take(self, n: int):
+        """(view, address) of the next n counts."""
+        if self._at + n > self._buf.array.size:
+            raise ValueError(f"count pool exhausted: {n} more counts, {self._buf.array.size - self._at} left")
+        view = self._buf.array[self._at:self._at + n]
+        addr = self._buf.ptr + self._at * 2
+        self._at += n
+        return view, addr
+
+    def close(self):
+        self._buf.close()
+
+
 @dataclass
 class Frame:
     """One frame's ROIs (host numpy arrays)."""
@@ -58,15 +83,21 @@ class Frame:
 
 
 class Request:
-    """A frame copied into pinned buffers, with pinned outputs."""
+    """A frame copied into pinned buffers, with pinned outputs.  With a pool,
+    the raw counts go into the pool's next slice (the pool owns them)."""
 
-    def __init__(self, frame: Frame):
+    def __init__(self, frame: Frame, pool: CountPool | None = None):
         n = frame.n
         self.frame = frame
         self._bufs = {}
+        ptrs = {}
         for name, arr in (("models", frame.models.astype(np.int8)), ("boxes", frame.boxes.astype(np.int8)),
                           ("xc", frame.xc.astype(np.int32)), ("yc", frame.yc.astype(np.int32)),
                           ("adu", frame.adu.astype(np.uint16))):
+            if name == "adu" and pool is not None:
+                view, ptrs[name] = pool.take(arr.size)
+                view[...] = arr
+                continue
             b = PinnedArray(arr.shape, arr.dtype)
             b.array[...] = arr
             self._bufs[name] = b
@@ -75,6 +106,7 @@ class Request:
             b.array[...] = np.nan
             self._bufs[name] = b
         p = {k: v.ptr for k, v in self._bufs.items()}
+        p.update(ptrs)
         self.struct = _lib.Request(n, p["models"], p["boxes"], p["xc"], p["yc"], p["adu"], p["params"], p["crlb"],
                                    p["llr"])
 
@@ -144,10 +176,11 @@ class Service:
         return {"slot_rois": a.value, "slot_pixels": b.value, "slots": c.value, "batches": d.value}
 
     def stats(self) -> dict:
-        v = np.zeros(10)
-        _lib.check(_lib.load().gmle_service_stats(self.handle, v.ctypes.data_as(_lib._DP), 10))
+        v = np.zeros(12)
+        _lib.check(_lib.load().gmle_service_stats(self.handle, v.ctypes.data_as(_lib._DP), 12))
         keys = ("batches", "rois_per_batch", "requests_per_batch", "host_launch_us", "gpu_us", "launch_to_seen_us",
-                "launch_to_start_us", "gpu_max_us", "launch_to_seen_max_us", "batches_staged")
+                "launch_to_start_us", "gpu_max_us", "launch_to_seen_max_us", "batches_staged", "dma_fraction",
+                "dma_copies_per_batch")
         return {k: round(float(x), 2) for k, x in zip(keys, v)}
 
     def close(self):

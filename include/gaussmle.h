@@ -298,8 +298,8 @@ void gmle_philox4x32_10(const uint32_t* counter, const uint32_t* key, uint32_t* 
  * counts) as they arrive:
  *   - a dispatcher thread packs every pending request that fits into a free
  *     slot's arena (admission by the HBM budget fixed at creation) and
- *     launches the slot's HIP graph: ingest (reads the requests' pinned
- *     buffers directly, converts ADU to electrons with the resident maps),
+ *     launches the slot's HIP graph: ingest (brings the requests' raw counts
+ *     to the GPU, converts ADU to electrons with the resident maps),
  *     grouping by shape, the fit (Float32: one multi-shape kernel for every
  *     served shape; mixed precision: one kernel per shape and its Float64
  *     final pass), result scatter (writes straight into each request's
@@ -311,6 +311,13 @@ void gmle_philox4x32_10(const uint32_t* counter, const uint32_t* key, uint32_t* 
  *     batch (each ROI is fitted alone, by its shape's kernel).
  * Request buffers must come from gmle_host_alloc (pinned, device-visible) and
  * stay untouched until the request completes.
+ * Raw counts reach the GPU by DMA for every run of a batch's requests whose
+ * counts lie back to back in host memory and add up to at least 1 MiB (one
+ * copy per run, at the copy engine's rate), and are read in place by the
+ * ingest kernel otherwise.  A caller that carves consecutive frames' counts
+ * out of one gmle_host_alloc block (a frame ring) gets one copy per batch.
+ * GMLE_SERVICE_INGEST=direct|dma|staged|auto overrides the choice (dma: every
+ * run by DMA; staged: packed on the host, then one DMA; direct: in place).
  * ------------------------------------------------------------------------- */
 typedef struct gmle_service gmle_service;
 
@@ -366,9 +373,8 @@ int gmle_service_info(gmle_service* s, int64_t* slot_rois, int64_t* slot_pixels,
  * a batch (us), time from launch to the completion being seen (us), time from
  * launch to the batch's start on the GPU being seen (us), largest GPU time of
  * a batch (us), largest launch-to-completion time (us), batches whose raw
- * counts were staged (packed on the host and copied by one DMA, with
- * GMLE_SERVICE_INGEST=staged, or =auto for batches of at least 1 MiB of
- * counts; by default the ingest kernel reads them in place).  A
+ * counts were staged (GMLE_SERVICE_INGEST=staged), the fraction of raw-count
+ * bytes that reached the GPU by DMA, and host-to-device copies per batch.  A
  * batch stops growing at 65,536 ROIs (a larger request forms a batch of its
  * own). */
 int gmle_service_stats(gmle_service* s, double* out, int n);

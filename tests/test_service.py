@@ -150,17 +150,21 @@ def test_service_mixed_precision_matches_batch_path():
         s.close()
 
 
+@pytest.mark.parametrize("pooled", [False, True], ids=["own_buffers", "count_pool"])
 @pytest.mark.parametrize("mode", ["direct", "staged", "dma", "auto"])
-def test_service_ingest_modes_give_identical_results(mode, monkeypatch):
+def test_service_ingest_modes_give_identical_results(mode, pooled, monkeypatch):
     """Raw counts read in place by the ingest kernel, packed on the host into
-    the slot's staging buffer and copied by one DMA, copied per request, or
-    chosen per batch by size: the pixels the fit sees are the same, so the
-    results are too."""
+    the slot's staging buffer and copied by one DMA, copied by one DMA per run
+    of requests whose counts are adjacent in host memory, or per run by its
+    size: the pixels the fit sees are the same, so the results are too.  With
+    the counts in one pool, a batch's requests form one run, and dma ingest
+    moves each batch with one copy."""
     maps, frs = frames()
     monkeypatch.setenv("GMLE_SERVICE_INGEST", mode)
     psfs, cons, table = sv.default_setup(lanes=LANE_TABLES["lanes_1_2_4"])
     s = sv.Service(psfs, cons, table, hbm_budget=1 << 30, slots=2)
-    reqs = [sv.Request(f) for f in frs]
+    pool = sv.CountPool(sum(f.adu.size for f in frs)) if pooled else None
+    reqs = [sv.Request(f, pool=pool) for f in frs]
     try:
         s.calibrate(*maps)
         want = split(reference(s, frs), frs)
@@ -168,12 +172,26 @@ def test_service_ingest_modes_give_identical_results(mode, monkeypatch):
             s.wait(t)
         for i, r in enumerate(reqs):
             assert_same(r.results(), want[i], f"{mode} request {i}")
-        staged = s.stats()["batches_staged"]
+        st = s.stats()
+        print(f"[ingest] {mode} pooled={pooled} {st}", flush=True)
         if mode == "staged":
-            assert staged > 0
+            assert st["batches_staged"] > 0
         elif mode != "auto":
-            assert staged == 0, (mode, staged)
+            assert st["batches_staged"] == 0, (mode, st)
+        if mode == "direct":
+            assert st["dma_fraction"] == 0.0 and st["dma_copies_per_batch"] == 0.0, st
+        if mode == "dma":
+            assert st["dma_fraction"] == 1.0, st
+            if pooled:
+                assert st["dma_copies_per_batch"] == 1.0, st   # one run per batch
+            else:
+                assert st["dma_copies_per_batch"] == st["requests_per_batch"], st
+        if mode == "auto" and not pooled:
+            # every request of this sensor is far below 1 MiB of counts
+            assert st["dma_fraction"] == 0.0, st
     finally:
         for r in reqs:
             r.close()
+        if pool is not None:
+            pool.close()
         s.close()
