@@ -172,13 +172,15 @@ struct ResultArgs {
 
 // One block per chunk of kChunk ROIs: lanes t < count load the ROIs'
 // metadata (box sizes clamped, so a request rewritten after submission can
-// never move a write outside its slot), then each wave converts whole ROIs
-// (wave w takes ROIs w, w + 4, ...), all of an ROI's pixels in flight at once.
+// never move a write outside its slot), then the whole block converts the
+// chunk's pixels, thread t taking pixels t, t + kIngestBlock, ..., so every
+// thread's map gathers are in flight together.
 __global__ __launch_bounds__(kIngestBlock) void service_ingest(IngestArgs a) {
   __shared__ int sbox[kChunk];
   __shared__ int sxc[kChunk];
   __shared__ int syc[kChunk];
   __shared__ int spre[kChunk + 1];
+  __shared__ float sinv[kChunk];
   __shared__ uint4 sraw[kRawVecs];
   const int t = threadIdx.x;
   for (int64_t ci = blockIdx.x; ci < a.n_chunks; ci += gridDim.x) {
@@ -200,6 +202,7 @@ __global__ __launch_bounds__(kIngestBlock) void service_ingest(IngestArgs a) {
     const int b = min(max((int)r.boxes[k], kMinBox), kMaxBatchBox);
     const int x = r.xc[k], y = r.yc[k];
     sbox[t] = b;
+    sinv[t] = 1.0f / (float)b;
     sxc[t] = x;
     syc[t] = y;
     a.models[roi] = r.models[k];
@@ -218,23 +221,25 @@ __global__ __launch_bounds__(kIngestBlock) void service_ingest(IngestArgs a) {
   }
   __syncthreads();
   if (t < c.count) a.offsets[r.roi_base + c.first + t] = r.pix_base + c.pix0 + spre[t];
-  const int wave = t / kWave, lane = t % kWave;
-  for (int q = wave; q < c.count; q += kIngestBlock / kWave) {
-    const int b = sbox[q];
-    const int p0 = spre[q];
-    if (p0 + b * b > npix) break;   // validated sizes never get here
-    float* dst = a.data + r.pix_base + c.pix0 + p0;
-    const int64_t m0 = (int64_t)(syc[q] - 1) + (int64_t)a.ny * (int64_t)(sxc[q] - 1);
+  // Pixels past npix (the chunk's validated count) are never written: with
+  // validated box sizes spre[count] == npix.
+  const int tot = min(spre[c.count], npix);
+  float* dst = a.data + r.pix_base + c.pix0;
+  for (int p = t; p < tot; p += kIngestBlock) {
+    // the pixel's ROI: the last q < count with spre[q] <= p
+    int q = 0;
 #pragma unroll
-    for (int u = 0; u < (kMaxBatchBox * kMaxBatchBox + kWave - 1) / kWave; ++u) {
-      const int p = lane + u * kWave;
-      if (p < b * b) {
-        const int i = p % b, j = p / b;   // row, column inside the ROI
-        int64_t m = m0 + i + (int64_t)a.ny * j;
-        m = m < 0 ? 0 : (m >= a.map_size ? a.map_size - 1 : m);   // corners are validated; never read outside
-        dst[p] = ((float)raw[p0 + p] - a.offset_map[m]) * a.gain_map[m];
-      }
-    }
+    for (int step = kChunk / 2; step > 0; step >>= 1)
+      if (q + step < c.count && spre[q + step] <= p) q += step;
+    const int b = sbox[q];
+    const int pq = p - spre[q];
+    // column j = floor(pq / b): (pq + 1/2) / b lies at least 1/(2b) >= 1/32
+    // from an integer and pq < 256, so the Float32 product is exact enough
+    const int j = (int)(((float)pq + 0.5f) * sinv[q]);
+    const int i = pq - j * b;   // row inside the ROI
+    int64_t m = (int64_t)(syc[q] - 1) + (int64_t)a.ny * (int64_t)(sxc[q] - 1 + j) + i;
+    m = m < 0 ? 0 : (m >= a.map_size ? a.map_size - 1 : m);   // corners are validated; never read outside
+    dst[p] = ((float)raw[p] - a.offset_map[m]) * a.gain_map[m];
   }
   __syncthreads();   // LDS is reused by the next chunk
   }
