@@ -27,13 +27,18 @@ constexpr int kDppSwap2 = 0x4E;        // quad_perm [2,3,0,1]
 constexpr int kDppHalfMirror = 0x141;  // row_half_mirror
 constexpr int kDppMirror = 0x140;      // row_mirror
 
+// bound_ctrl is set: a lane whose source falls outside its row reads 0
+// instead of an undefined value.  Only row_shr in group_next<8> has such lanes,
+// and their values are never selected.  With it set, the compiler folds the
+// move into the add or subtraction that reads it (v_add_f32_dpp): one VALU
+// instruction fewer per reduction step.
 template <int CTRL> __device__ __forceinline__ float dpp_move(float v) {
-  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
 }
 template <int CTRL> __device__ __forceinline__ double dpp_move(double v) {
   const int64_t b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffff), CTRL, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, false);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffff), CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, true);
   return __longlong_as_double(((int64_t)hi << 32) | (uint32_t)lo);
 }
 __device__ __forceinline__ float dpp_swap1(float v) { return dpp_move<kDppSwap1>(v); }
