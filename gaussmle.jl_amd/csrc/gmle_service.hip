@@ -515,6 +515,13 @@ int build_slot(gmle_service* s, Slot& sl, size_t slot_bytes) {
   sl.chunks_h = (Chunk*)h;
   sl.chunks_d = (Chunk*)d;
   if ((e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking)) != hipSuccess) return hip_fail(e, "hipStreamCreate");
+  // The first host-to-device copy on a stream takes milliseconds of set-up:
+  // pay it here (the pinned request table into the raw area) rather than in
+  // the first batch whose counts arrive by DMA.
+  const size_t warm = std::min(sizeof(ReqDesc) * (size_t)kMaxReqPerBatch, (size_t)s->cap_pix * sizeof(uint16_t));
+  if ((e = hipMemcpyAsync(sl.raw, sl.reqs_h, warm, hipMemcpyHostToDevice, sl.stream)) != hipSuccess ||
+      (e = hipStreamSynchronize(sl.stream)) != hipSuccess)
+    return hip_fail(e, "first copy on the slot's stream");
   if ((e = hipEventCreate(&sl.start)) != hipSuccess) return hip_fail(e, "hipEventCreate");
   if ((e = hipEventCreate(&sl.done)) != hipSuccess) return hip_fail(e, "hipEventCreate");
 
