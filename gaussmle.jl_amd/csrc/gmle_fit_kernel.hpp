@@ -1344,11 +1344,28 @@ __device__ __forceinline__ void fit_one(const FitArgs& a, const RoiRef& ref, boo
       H[k] = group_sum<L>(H[k]);
     }
     // Scalar Newton step with constraints (unified_kernel.jl:337-350, constraints.jl:172-177).
+    // G / H is a v_rcp_f32 and a multiply (1 ulp more than the division) where
+    // both are below 1e30 in magnitude; a lane with a NaN, an infinity or a
+    // value beyond that takes the IEEE division, so those cases come out as
+    // the division gives them.  The division runs only in a wave that has
+    // such a lane.
+    float step[P];
+    bool exact = true;
 #pragma unroll
     for (int k = 0; k < P; ++k) {
-      const float step = fabsf(H[k]) > 1e-10f ? G[k] / H[k] : 0.f;
+      const float g = G[k], h = H[k];
+      step[k] = fabsf(h) > 1e-10f ? g * __builtin_amdgcn_rcpf(h) : 0.f;
+      exact = exact && fabsf(g) < 1e30f && fabsf(h) < 1e30f;
+    }
+    if (__builtin_amdgcn_ballot_w64(!exact) != 0) {
+#pragma unroll
+      for (int k = 0; k < P; ++k)
+        if (!exact) step[k] = fabsf(H[k]) > 1e-10f ? G[k] / H[k] : 0.f;
+    }
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
       const float ms = a.max_step[k];
-      const float t = th[k] - jclamp(step, -ms, ms);
+      const float t = th[k] - jclamp(step[k], -ms, ms);
       th[k] = jclamp(t, a.lower[k], a.upper[k]);
     }
   }
