@@ -820,28 +820,33 @@ template <class TF, int P> struct FinalOut {
 // upper triangle Fu.  du holds the pixel's P model derivatives.
 template <int CAM, int P, class TF>
 __device__ __forceinline__ void final_pixel(TF d, TF var, TF mu, const TF* du, bool cvalid, TF* Fu, TF& llr) {
+  constexpr bool F32 = sizeof(TF) == 4;
   TF v = mu;
   TF lp;
-  TF rmu = TF(0);   // 1/mu, Float64 ideal camera only
+  TF rv = TF(0);   // 1/mu, the ideal camera's Fisher weight
   if constexpr (CAM == IDEAL) {
-    // camera_models.jl:60-71; in Float64 as d log(mu/d) + d - mu (one log)
-    if constexpr (sizeof(TF) == 8) {
-      // d log(mu/d) = -d log(d (1/mu)); 1/mu is also the Fisher weight below
-      rmu = mu > TF(0) ? TF(1) / mu : TF(0);
-      lp = (mu > TF(0) && d > TF(0)) ? -d * log(d * rmu) + (d - mu) : (mu > TF(0) && d == TF(0)) ? -mu : TF(0);
+    // camera_models.jl:60-71, as d log(mu/d) + d - mu with one log:
+    // d log(mu/d) = -d log(d (1/mu)), and 1/mu is also the Fisher weight below.
+    // Float32 takes 1/mu from v_rcp_f32 (1 ulp); the single log also avoids
+    // the cancellation of d log(mu) against d log(d) in the two-log form.
+    if constexpr (F32) {
+      rv = mu > TF(0) ? fast_rcp(mu) : TF(0);
+      lp = (mu > TF(0) && d > TF(0)) ? -d * logf(d * rv) + (d - mu) : (mu > TF(0) && d == TF(0)) ? -mu : TF(0);
     } else {
-      lp = (mu > TF(0) && d > TF(0)) ? d * logf(mu) - mu - (d * logf(d) - d)
-                                    : (mu > TF(0) && d == TF(0)) ? -mu : TF(0);
+      rv = mu > TF(0) ? TF(1) / mu : TF(0);
+      lp = (mu > TF(0) && d > TF(0)) ? -d * log(d * rv) + (d - mu) : (mu > TF(0) && d == TF(0)) ? -mu : TF(0);
     }
   } else {
-    // camera_models.jl:73-85
+    // camera_models.jl:73-85.  The Fisher weight below divides again, inside
+    // its own block: a 1/v shared by both keeps it live across the block, and
+    // the multi-shape kernel's largest sCMOS tiles then exceed 256 VGPRs.
     v = mu + var;
     const TF res = d - mu;
     lp = v > TF(0) ? TF(-0.5) * (res * res / v) : TF(0);
   }
   if (cvalid) llr += lp;
   if (cvalid && mu > TF(0)) {
-    const TF wv = (CAM == IDEAL && sizeof(TF) == 8) ? rmu : TF(1) / v;
+    const TF wv = CAM == IDEAL ? rv : TF(1) / v;
     int q = 0;
 #pragma unroll
     for (int k = 0; k < P; ++k) {
