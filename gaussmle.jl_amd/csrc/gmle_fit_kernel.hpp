@@ -830,8 +830,13 @@ __device__ __forceinline__ void final_pixel(TF d, TF var, TF mu, const TF* du, b
     // Float32 takes 1/mu from v_rcp_f32 (1 ulp); the single log also avoids
     // the cancellation of d log(mu) against d log(d) in the two-log form.
     if constexpr (F32) {
+      // ln x as v_log_f32 (log2) times ln 2.  x = d/mu of a pixel with d > 0
+      // lies far above the denormal range; the clamp keeps even a denormal
+      // product finite (its term is then ~0, as it would be).
       rv = mu > TF(0) ? fast_rcp(mu) : TF(0);
-      lp = (mu > TF(0) && d > TF(0)) ? -d * logf(d * rv) + (d - mu) : (mu > TF(0) && d == TF(0)) ? -mu : TF(0);
+      const float x = fmaxf(d * rv, 1.17549435e-38f);
+      const float lnx = __builtin_amdgcn_logf(x) * 0.693147180559945309f;
+      lp = (mu > TF(0) && d > TF(0)) ? -d * lnx + (d - mu) : (mu > TF(0) && d == TF(0)) ? -mu : TF(0);
     } else {
       rv = mu > TF(0) ? TF(1) / mu : TF(0);
       lp = (mu > TF(0) && d > TF(0)) ? -d * log(d * rv) + (d - mu) : (mu > TF(0) && d == TF(0)) ? -mu : TF(0);
