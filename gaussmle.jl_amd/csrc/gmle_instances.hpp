@@ -81,6 +81,25 @@ Fn pick(int box, int lanes) {
 
 }  // namespace gmle
 
+// Specialisations compiled in a translation unit of their own, under another
+// machine scheduler: that unit instantiates the launcher and the description
+// (GMLE_INSTANTIATE_FIT), and the finder's unit declares them extern
+// (GMLE_EXTERN_FIT, before GMLE_DEFINE_FINDER) so that it builds neither them
+// nor their kernel.  Float32 only: a mixed-precision launcher also launches
+// the Float64 final pass, which its finder's unit builds as well.
+#define GMLE_EXTERN_FIT(MODEL, CAM, PREC, N, L)                                                 \
+  namespace gmle {                                                                             \
+  static_assert(PREC == PREC_FP32, "only Float32 launchers are built apart");                  \
+  extern template hipError_t launch_fit<MODEL, CAM, N, L, PREC>(const FitArgs&, hipStream_t);   \
+  extern template bool describe_fit<MODEL, CAM, N, L, PREC>(KernelInfo*);                       \
+  }
+#define GMLE_INSTANTIATE_FIT(MODEL, CAM, PREC, N, L)                                            \
+  namespace gmle {                                                                             \
+  static_assert(PREC == PREC_FP32, "only Float32 launchers are built apart");                  \
+  template hipError_t launch_fit<MODEL, CAM, N, L, PREC>(const FitArgs&, hipStream_t);          \
+  template bool describe_fit<MODEL, CAM, N, L, PREC>(KernelInfo*);                              \
+  }
+
 // One translation unit per (model, camera, precision) defines
 //   LaunchFn   find_launcher_<MODEL>_<CAM>_<PREC>(int box, int lanes)
 //   DescribeFn find_describe_<MODEL>_<CAM>_<PREC>(int box, int lanes)
