@@ -40,11 +40,12 @@ HIPFLAGS = [
     "-Wno-unused-but-set-variable",
 ]
 
-# Per-file flags.  inst_sched_ilp.hip holds the 4-lane specialisations that run
-# faster under the iterative ILP machine scheduler (the file lists them and
-# the measurements).
+# Per-file flags.  inst_sched_ilp.hip and inst_sched_maxilp.hip hold the
+# specialisations that run faster under another machine scheduler (each file
+# lists them and the measurements).
 SOURCE_FLAGS = {
     "inst_sched_ilp.hip": ["-mllvm", "-amdgpu-sched-strategy=iterative-ilp"],
+    "inst_sched_maxilp.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp"],
 }
 
 
