@@ -100,6 +100,17 @@ Fn pick(int box, int lanes) {
   template bool describe_fit<MODEL, CAM, N, L, PREC>(KernelInfo*);                              \
   }
 
+// The same for the iteration kernel alone, for mixed precision: the launcher
+// and the Float64 final pass stay in the finder's unit.
+#define GMLE_EXTERN_KERNEL(MODEL, CAM, PREC, N, L)                                              \
+  namespace gmle {                                                                             \
+  extern template __global__ void fit_kernel<MODEL, CAM, N, L, PREC>(FitArgs);                  \
+  }
+#define GMLE_INSTANTIATE_KERNEL(MODEL, CAM, PREC, N, L)                                         \
+  namespace gmle {                                                                             \
+  template __global__ void fit_kernel<MODEL, CAM, N, L, PREC>(FitArgs);                         \
+  }
+
 // One translation unit per (model, camera, precision) defines
 //   LaunchFn   find_launcher_<MODEL>_<CAM>_<PREC>(int box, int lanes)
 //   DescribeFn find_describe_<MODEL>_<CAM>_<PREC>(int box, int lanes)
