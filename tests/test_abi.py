@@ -135,3 +135,42 @@ def test_fit_without_device_reports_it(hiplib):
     from gaussmle_amd import _lib
     d = np.ones(11 * 11 * 4, np.float32)
     assert _call_fit(hiplib, d, 11, 4) == _lib.GMLE_ERR_NODEVICE
+
+
+CSRC = ROOT / "gaussmle.jl_amd" / "csrc"
+_SPEC = r"\(\s*(\w+)\s*,\s*(\w+)\s*,\s*(\w+)\s*,\s*(\d+)\s*,\s*(\d+)\s*\)"
+
+
+def _specs(kind: str, files):
+    out = []
+    for f in files:
+        for m in re.finditer(rf"^GMLE_{kind}_(FIT|KERNEL){_SPEC}", f.read_text(), flags=re.M):
+            out.append((m.group(1),) + m.groups()[1:])
+    return out
+
+
+def test_kernels_built_apart_are_instantiated_once():
+    """A finder declares some specialisations extern (GMLE_EXTERN_FIT /
+    GMLE_EXTERN_KERNEL); each must be instantiated, with the same macro kind,
+    in exactly one of the inst_sched_*.hip units, and each of those units has
+    its own compiler flags in build.py."""
+    sched = sorted(CSRC.glob("inst_sched_*.hip"))
+    finders = [f for f in sorted(CSRC.glob("inst_*.hip")) if f not in sched]
+    externs = _specs("EXTERN", finders)
+    inst = _specs("INSTANTIATE", sched)
+    assert externs, "no specialisation is built apart"
+    assert sorted(externs) == sorted(inst)
+    assert len(set(inst)) == len(inst)
+    assert not _specs("INSTANTIATE", finders) and not _specs("EXTERN", sched)
+    build = (ROOT / "gaussmle.jl_amd" / "build.py").read_text()
+    for f in sched:
+        assert f'"{f.name}": ["-mllvm", "-amdgpu-sched-strategy=' in build, f.name
+
+
+def test_library_leaves_no_gmle_symbol_undefined(hiplib):
+    """A kernel declared extern but built nowhere would be an undefined
+    symbol of the shared library (-shared links it anyway)."""
+    from gaussmle_amd import _lib
+    out = subprocess.run(["nm", "-D", "-C", "--undefined-only", str(_lib.LIB_PATH)], capture_output=True, text=True,
+                         check=True).stdout
+    assert "gmle::" not in out, [line for line in out.splitlines() if "gmle::" in line]
