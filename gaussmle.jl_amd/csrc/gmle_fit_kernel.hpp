@@ -1338,11 +1338,19 @@ __device__ __forceinline__ void fit_one(const FitArgs& a, const RoiRef& ref, boo
         }
       }
 
+      // One lane per ROI: the pixel sums run at the normal wave priority and
+      // the serial rest of the iteration (the Newton step, the next
+      // iteration's widths and tables) at a raised one, so that the SIMD's
+      // other wave fills that part's stalls with its pixel work rather than
+      // the two waves reaching the serial part together.  (gfx950, XYNB 11 px,
+      // 1M ROIs: 1.403 -> 1.393 ms per launch.)
+      if constexpr (L == 1) __builtin_amdgcn_s_setprio(0);
       if (fast) {
         accumulate<MODEL, CAM, N, L, true>(dv, vr, c0, nph, bg, ry, kc, w, Ay, AAy, sphi, sD, sD2, sA, sAA, G, H);
       } else {
         accumulate<MODEL, CAM, N, L, false>(dv, vr, c0, nph, bg, ry, kc, w, Ay, AAy, sphi, sD, sD2, sA, sAA, G, H);
       }
+      if constexpr (L == 1) __builtin_amdgcn_s_setprio(1);
       if constexpr (TAIL) {
 #pragma unroll
         for (int k = 0; k < P; ++k) { G[k] += Gt[k]; H[k] += Ht[k]; }
@@ -1380,6 +1388,7 @@ __device__ __forceinline__ void fit_one(const FitArgs& a, const RoiRef& ref, boo
     }
   }
 
+  if constexpr (L == 1) __builtin_amdgcn_s_setprio(0);
   // ---- final pass: LLR + Fisher + CRLB (unified_kernel.jl:353-433) --------
   // Float32 here; the Float64 final pass runs as its own kernel
   // (final_pass_kernel) so that its registers do not lower this kernel's
