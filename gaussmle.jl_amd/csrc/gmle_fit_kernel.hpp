@@ -1274,6 +1274,7 @@ __device__ __forceinline__ void fit_one(const FitArgs& a, const RoiRef& ref, boo
         accumulate_rows<MODEL, CAM, N, L, false, SHARE>(dv, vr, dt, vt, c0, nph, bg, kr, r, kc, w, G, H);
       }
     } else {
+      if constexpr (L == 1) __builtin_amdgcn_s_setprio(0);   // tables and pixel sums: normal priority
       AxisTab<float, N> ry;
       const AxisK<float> kc = axis_consts<float>(th[Tr::IX], w.sx, nph);
       // tail split: the last column's rows r + L s, with this lane's own row
@@ -1338,12 +1339,14 @@ __device__ __forceinline__ void fit_one(const FitArgs& a, const RoiRef& ref, boo
         }
       }
 
-      // One lane per ROI: the pixel sums run at the normal wave priority and
-      // the serial rest of the iteration (the Newton step, the next
-      // iteration's widths and tables) at a raised one, so that the SIMD's
-      // other wave fills that part's stalls with its pixel work rather than
-      // the two waves reaching the serial part together.  (gfx950, XYNB 11 px,
-      // 1M ROIs: 1.403 -> 1.393 ms per launch.)
+      // One lane per ROI: the edge tables and pixel sums run at the normal
+      // wave priority, and the short dependent chain between them (the Newton
+      // step, then the next iteration's widths and axis constants) at a
+      // raised one, so that the SIMD's other wave fills that chain's stalls
+      // with its own pixel work.  (gfx950, XYNB 11 px, 1M ROIs: 1.403 ->
+      // 1.374 ms per launch; raising the tables too gave 1.393, raising the
+      // pixel sums instead 1.431.)  Setting priority 0 again here as well
+      // keeps the compiler's scheduling regions as measured.
       if constexpr (L == 1) __builtin_amdgcn_s_setprio(0);
       if (fast) {
         accumulate<MODEL, CAM, N, L, true>(dv, vr, c0, nph, bg, ry, kc, w, Ay, AAy, sphi, sD, sD2, sA, sAA, G, H);
